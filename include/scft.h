@@ -1,0 +1,154 @@
+/*
+ * scft.h — C ABI of libscft, an MI355X (gfx950) implementation of the
+ * iteration loop of the pseudo-spectral SCFT solver SCFT.c
+ * (gahme-glitch/stimuli-responsive-nanodroplets).
+ *
+ * The reference has no library boundary: its loop body calls file-static
+ * C functions from main().  Each entry point below names the reference
+ * code it replaces.  Conventions of this ABI:
+ *   - every function returns int: 0 = ok, < 0 = error (SCFT_E*);
+ *   - plain pointers and sizes only, host memory on the host side;
+ *   - a context owns its device memory and one HIP stream; contexts are
+ *     independent, so two threads may drive two contexts;
+ *   - grid arrays use the reference index idx = k + NZ*(j + NY*i)
+ *     (z fastest), field arrays are [field][idx] with field order
+ *     A..J, S1, S2, S3 (0..12) as in SCFT.c:2001-2015.
+ */
+#ifndef SCFT_H
+#define SCFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCFT_NFIELDS   13   /* A..J, S1..S3 */
+#define SCFT_NSPECIES  17   /* dbk1..4, homo A..J, S1..S3 (energies.dat order, SCFT.c:1586-1589) */
+
+enum {
+    SCFT_OK          = 0,
+    SCFT_EINVAL      = -1,  /* bad argument */
+    SCFT_EHIP        = -2,  /* a HIP runtime call failed */
+    SCFT_ENOMEM      = -3,  /* device allocation failed */
+    SCFT_EVOLUME     = -4,  /* free-chain volumes do not sum to 1 (SCFT.c:84-101) */
+    SCFT_EUNSUPPORTED= -5,  /* grid/transform size outside what the kernels support */
+    SCFT_ENONFINITE  = -6   /* a NaN/Inf appeared in a reduction */
+};
+
+enum {
+    SCFT_FP64       = 0,    /* everything in double precision */
+    SCFT_FP32_SPLIT = 1     /* FP32 propagators/FFTs/contour sums (Kahan), FP64 Q, F, update */
+};
+
+/*
+ * Run parameters: the macros of parameters.h:1-179 as a runtime struct.
+ * chi is the symmetric CHI_N matrix over fields A..J,S1..S3 before the
+ * zeroing of SCFT.c:303-396 (the diagonal is ignored).
+ */
+typedef struct scft_params {
+    double lx, ly, lz;          /* LX, LY, LZ (R_g)                     parameters.h:2-4   */
+    double dr_target;           /* DR_TARGET                            parameters.h:19    */
+    int    ns;                  /* NS                                   parameters.h:18    */
+    int    mask;                /* MASK: 0 periodic, 1 Dirichlet z      parameters.h:11    */
+    double g1;                  /* G1                                   parameters.h:21    */
+    double dt_noise;            /* DT_NOISE                             parameters.h:10    */
+    unsigned long relaxation;   /* RELAXATION                           parameters.h:8     */
+    int    enable_seed;         /* ENABLE_SEED                          parameters.h:14    */
+    int    seed_steps;          /* SEED_STEPS                           parameters.h:15    */
+    int    seed_area_delta;     /* SEED_AREA_DELTA                      parameters.h:16    */
+    double chi[SCFT_NFIELDS][SCFT_NFIELDS];
+    double f_dbk[4], len_dbk[4], vol_dbk[4];      /* parameters.h:77-87   */
+    double vol_homo[10], len_homo[10];            /* parameters.h:89-109  */
+    double vol_s[3], len_s[3];                    /* parameters.h:112-119 */
+} scft_params;
+
+/* Quantities the setup of SCFT.c:84-396, 718-734 derives from the parameters. */
+typedef struct scft_derived {
+    int    nx, ny, nz;                 /* SCFT.c:103-105 */
+    int    nzext;                      /* N = 2(NZ+1) for MASK 1, else NZ (SCFT.c:680) */
+    int    ns_sp[SCFT_NSPECIES];       /* NS per species, 0 = absent (SCFT.c:121-288) */
+    int    ns_x[4], ns_y[4];           /* diblock block lengths */
+    double alpha[SCFT_NSPECIES];       /* (NS_p - 1)*DS */
+    double phi_avg[SCFT_NFIELDS];      /* phi_avgs[] after the coating rescale */
+    double chi_eff[SCFT_NFIELDS][SCFT_NFIELDS];
+    double coating_vol;
+    int    active[SCFT_NFIELDS];       /* update-active fields (SCFT.c:2017-2042) */
+    int    n_active;
+} scft_derived;
+
+/* Per-iteration scalars (all FP64). */
+typedef struct scft_iter_out {
+    double free_energy;                /* H_fe after the ln Q terms (SCFT.c:1525-1555) */
+    double rmsdev;                     /* SCFT.c:1564-1580 */
+    double dw_rms;                     /* sqrt(mean over active fields and voxels of (w_new - w_old)^2) */
+    double qsum[SCFT_NSPECIES];        /* Q_p (0 for absent species) */
+    double chempot[SCFT_NSPECIES];     /* -ln(Q_p^2)/2, 0 if absent (SCFT.c:1862-1869) */
+    float  mf;                         /* MF (SCFT.c:1515-1522) */
+} scft_iter_out;
+
+/* Setup derivation only (host logic, no device). Replaces SCFT.c:84-396, 718-734. */
+int scft_derive(const scft_params *p, scft_derived *out);
+
+typedef struct scft_ctx scft_ctx;
+
+/* Allocate device state for one SCFT instance on the current HIP device.
+ * Replaces the allocation and planning of SCFT.c:433-701 and the
+ * propagator start slices / k-space operator of SCFT.c:881-1023. */
+int scft_create(const scft_params *p, int precision, scft_ctx **out);
+int scft_destroy(scft_ctx *ctx);
+int scft_get_derived(const scft_ctx *ctx, scft_derived *out);
+
+/* Field transfer (restart / seeding / snapshots; SCFT.c:844-866, 1601-1650).
+ * w: 13*V doubles [field][idx]; p: V doubles. */
+int scft_set_fields(scft_ctx *ctx, const double *w, const double *p);
+int scft_get_fields(scft_ctx *ctx, double *w, double *p);
+/* Brush start masks, 3*V doubles [S1,S2,S3][idx] (wMask.dat, SCFT.c:756-781). */
+int scft_set_mask(scft_ctx *ctx, const double *mask);
+/* Total densities, 13*V doubles [A_tot..J_tot, S1, S2, S3][idx], and phi_tot (V). */
+int scft_get_densities(scft_ctx *ctx, double *phi, double *phi_tot);
+/* One propagator slice s of species sp (0..16); which = 0 q, 1 q-dagger (V doubles). */
+int scft_get_propagator(scft_ctx *ctx, int sp, int which, int s, double *out);
+
+/*
+ * scft_propagate: every propagator of one iteration, in the species order
+ * and block bounds of SCFT.c:1092-1456 (funPDESolver / funPDESolverBrush,
+ * SCFT.c:2310-2472, with the periodic and Dirichlet steps of
+ * SCFT.c:2474-2631).  iter is the iteration number after the increment of
+ * SCFT.c:1090 (it selects the dbk1 seed start, SCFT.c:2346-2367).
+ */
+int scft_propagate(scft_ctx *ctx, unsigned long iter);
+
+/*
+ * scft_densities: Q_p and chemical potentials (funChemPotential,
+ * SCFT.c:1837-1872), q*q-dagger contour integration (funMultiplyQs +
+ * funDensity, SCFT.c:2243-2308), volume scaling and totals
+ * (funVolFrac/funAddArray, SCFT.c:1458-1513).  qsum/chempot: 17 doubles
+ * each, or NULL.
+ */
+int scft_densities(scft_ctx *ctx, double *qsum, double *chempot);
+
+/*
+ * scft_update_fields: free energy with pre-update fields (funFreeEnergy
+ * and the ln Q terms, SCFT.c:1525-1555, 1874-1917), the field/pressure
+ * update (funUpdate, SCFT.c:1993-2124) and RMSDev (SCFT.c:1564-1580).
+ * noise: NULL, or the Gaussian deviates in the reference's draw order
+ * (voxel k->j->i outer, active field inner), n_active*V doubles; it is
+ * used only when mf != 0 (amplitude dt_noise*mf, SCFT.c:2103).
+ */
+int scft_update_fields(scft_ctx *ctx, unsigned long iter, float mf,
+                       const double *noise, scft_iter_out *out);
+
+/* One whole iteration: propagate + densities + update (SCFT.c:1088-1580). */
+int scft_iterate(scft_ctx *ctx, unsigned long iter, float mf,
+                 const double *noise, scft_iter_out *out);
+
+/* Wait for the context's stream. */
+int scft_sync(scft_ctx *ctx);
+/* Last error text of this thread (static storage). */
+const char *scft_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCFT_H */
