@@ -143,6 +143,39 @@ int scft_update_fields(scft_ctx *ctx, unsigned long iter, float mf,
 int scft_iterate(scft_ctx *ctx, unsigned long iter, float mf,
                  const double *noise, scft_iter_out *out);
 
+/*
+ * Batches.  A batch context holds nslots independent SCFT instances that
+ * share one chain structure (species set, NS, lengths and block fractions,
+ * MASK, LY, LZ, DR_TARGET, and whether LX == 1); chi, volumes, G1 and LX
+ * (so NX <= max_nx) may differ per slot.  One scft_batch_iterate call
+ * advances every active slot by one iteration; the propagator launches of
+ * a contour step cover every chain of every active slot.  scft_create is a
+ * batch of one slot, and the single-instance calls above act on slot 0.
+ * Not in the reference (one process per SCFT run there); it serves the
+ * phase-diagram workflow of README.md:59-128, whose trial solves are
+ * independent SCFT runs.
+ */
+int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int precision, scft_ctx **out);
+int scft_batch_size(const scft_ctx *ctx);
+/* (Re)initialise a slot for a new parameter set: fields and densities
+ * zeroed, q[0] = 1; the slot is left inactive. */
+int scft_slot_init(scft_ctx *ctx, int slot, const scft_params *p);
+int scft_slot_activate(scft_ctx *ctx, int slot, int on);
+int scft_slot_get_derived(const scft_ctx *ctx, int slot, scft_derived *out);
+int scft_slot_set_fields(scft_ctx *ctx, int slot, const double *w, const double *p);
+int scft_slot_get_fields(scft_ctx *ctx, int slot, double *w, double *p);
+/* Device-to-device field copy (seed a trial solve from a converged slot). */
+int scft_slot_copy_fields(scft_ctx *ctx, int dst_slot, int src_slot);
+int scft_slot_set_mask(scft_ctx *ctx, int slot, const double *mask);
+int scft_slot_get_densities(scft_ctx *ctx, int slot, double *phi, double *phi_tot);
+int scft_slot_get_propagator(scft_ctx *ctx, int slot, int sp, int which, int s, double *out);
+/* One iteration of every active slot.  iters[slot] (or NULL: no seed box)
+ * selects the dbk1 seed start; outs[slot] is filled for active slots.
+ * No noise (MF = 0) in batch mode. */
+int scft_batch_iterate(scft_ctx *ctx, const unsigned long *iters, scft_iter_out *outs);
+/* Human-readable schedule summary (z operator kind, launches per iteration). */
+int scft_get_schedule(const scft_ctx *ctx, char *buf, int len);
+
 /* Wait for the context's stream. */
 int scft_sync(scft_ctx *ctx);
 /* Last error text of this thread (static storage). */

@@ -152,6 +152,19 @@ def load_library(path=None):
         "scft_iterate": ([vp, C.c_ulong, C.c_float, dp, P(ScftIterOut)], C.c_int),
         "scft_sync": ([vp], C.c_int),
         "scft_last_error": ([], C.c_char_p),
+        "scft_batch_create": ([P(ScftParams), C.c_int, C.c_int, C.c_int, P(vp)], C.c_int),
+        "scft_batch_size": ([vp], C.c_int),
+        "scft_slot_init": ([vp, C.c_int, P(ScftParams)], C.c_int),
+        "scft_slot_activate": ([vp, C.c_int, C.c_int], C.c_int),
+        "scft_slot_get_derived": ([vp, C.c_int, P(ScftDerived)], C.c_int),
+        "scft_slot_set_fields": ([vp, C.c_int, dp, dp], C.c_int),
+        "scft_slot_get_fields": ([vp, C.c_int, dp, dp], C.c_int),
+        "scft_slot_copy_fields": ([vp, C.c_int, C.c_int], C.c_int),
+        "scft_slot_set_mask": ([vp, C.c_int, dp], C.c_int),
+        "scft_slot_get_densities": ([vp, C.c_int, dp, dp], C.c_int),
+        "scft_slot_get_propagator": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, dp], C.c_int),
+        "scft_batch_iterate": ([vp, P(C.c_ulong), P(ScftIterOut)], C.c_int),
+        "scft_get_schedule": ([vp, C.c_char_p, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -242,3 +255,93 @@ class Context:
 
     def sync(self):
         check(self.lib, self.lib.scft_sync(self.h), "scft_sync")
+
+    def schedule(self):
+        buf = C.create_string_buffer(256)
+        check(self.lib, self.lib.scft_get_schedule(self.h, buf, 256), "scft_get_schedule")
+        return buf.value.decode()
+
+
+class Batch:
+    """A batch of SCFT instances (slots) that advance together on the GPU."""
+
+    def __init__(self, lib, proto, nslots, max_nx=0, precision=SCFT_FP64):
+        self.lib = lib
+        self.proto = proto
+        self.nslots = nslots
+        h = C.c_void_p()
+        check(lib, lib.scft_batch_create(C.byref(proto), nslots, max_nx, precision, C.byref(h)),
+              "scft_batch_create")
+        self.h = h
+        self.V = [0] * nslots
+        self.params = [None] * nslots
+        self._outs = (ScftIterOut * nslots)()
+        self._iters = (C.c_ulong * nslots)()
+
+    def close(self):
+        if self.h:
+            self.lib.scft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_slot(self, s, params):
+        check(self.lib, self.lib.scft_slot_init(self.h, s, C.byref(params)), "scft_slot_init")
+        d = ScftDerived()
+        check(self.lib, self.lib.scft_slot_get_derived(self.h, s, C.byref(d)), "scft_slot_get_derived")
+        self.V[s] = d.nx * d.ny * d.nz
+        self.params[s] = params
+        return d
+
+    def activate(self, s, on=True):
+        check(self.lib, self.lib.scft_slot_activate(self.h, s, 1 if on else 0), "scft_slot_activate")
+
+    def set_fields(self, s, w, p):
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(NF, self.V[s])
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(self.V[s])
+        check(self.lib, self.lib.scft_slot_set_fields(self.h, s, _ptr(w), _ptr(p)), "scft_slot_set_fields")
+
+    def get_fields(self, s):
+        w = np.zeros((NF, self.V[s]))
+        p = np.zeros(self.V[s])
+        check(self.lib, self.lib.scft_slot_get_fields(self.h, s, _ptr(w), _ptr(p)), "scft_slot_get_fields")
+        return w, p
+
+    def copy_fields(self, dst, src):
+        check(self.lib, self.lib.scft_slot_copy_fields(self.h, dst, src), "scft_slot_copy_fields")
+
+    def set_mask(self, s, m):
+        m = np.ascontiguousarray(m, dtype=np.float64).reshape(3, self.V[s])
+        check(self.lib, self.lib.scft_slot_set_mask(self.h, s, _ptr(m)), "scft_slot_set_mask")
+
+    def get_densities(self, s):
+        phi = np.zeros((NF, self.V[s]))
+        tot = np.zeros(self.V[s])
+        check(self.lib, self.lib.scft_slot_get_densities(self.h, s, _ptr(phi), _ptr(tot)),
+              "scft_slot_get_densities")
+        return phi, tot
+
+    def get_propagator(self, s, sp, which, idx):
+        out = np.zeros(self.V[s])
+        check(self.lib, self.lib.scft_slot_get_propagator(self.h, s, sp, which, idx, _ptr(out)),
+              "scft_slot_get_propagator")
+        return out
+
+    def iterate(self, iters=None):
+        """One iteration of every active slot; returns the out structs (by slot)."""
+        ip = None
+        if iters is not None:
+            for i, v in enumerate(iters):
+                self._iters[i] = v
+            ip = self._iters
+        check(self.lib, self.lib.scft_batch_iterate(self.h, ip, self._outs), "scft_batch_iterate")
+        return self._outs
+
+    def schedule(self):
+        buf = C.create_string_buffer(256)
+        check(self.lib, self.lib.scft_get_schedule(self.h, buf, 256), "scft_get_schedule")
+        return buf.value.decode()

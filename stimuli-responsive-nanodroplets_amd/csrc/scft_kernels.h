@@ -2,11 +2,20 @@
 //
 // Propagator: q_{s+1} = E . M (E . q_s) with E = exp(-DS w / 2) and
 // M = F^-1 diag(K) F / (NX NY N) (SCFT.c:2474-2631).  K is separable,
-// K = Kx Ky Kz, so M factors into one line operator per axis; each is one
-// LDS-resident FFT -> K -> inverse FFT over a tile of grid lines.  The
-// z operator with MASK 1 is the reference's Dirichlet transform: the line
-// is odd-extended to N = 2(NZ+1) (SCFT.c:2551-2592), transformed, scaled by
-// Kz on the extended grid, transformed back and its points 1..NZ kept.
+// K = Kx Ky Kz, so M factors into one real line operator per axis, applied
+// to every grid line of that axis:
+//   - k_lineop: LDS-resident Stockham FFT -> K -> inverse FFT over a tile of
+//     lines (two real lines packed per complex line).  With MASK 1 the z
+//     lines are odd-extended to N = 2(NZ+1) first, which is the reference's
+//     Dirichlet transform (SCFT.c:2551-2592).
+//   - k_zdense: the same z operator as an NZ x NZ matrix.  M commutes with
+//     the reversal k -> NZ-1-k, so it splits into a symmetric and an
+//     antisymmetric half of (NZ/2)^2 each: NZ^2/2 FMA per line, against
+//     the FFT of length 2(NZ+1) = 2*3*43 (NZ = 128) whose prime factor 43
+//     costs ~6x that.  The autotuner picks per size.
+// All transform arithmetic is FP64; propagator slices are stored as T
+// (double, or float for the FP32-split mode).  Every launch covers every
+// live chain of every active instance of the batch (grid.y = op index).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,37 +26,38 @@ namespace scft {
 
 constexpr int kThreads = 256;
 constexpr int kMaxTerms = 24;   // density terms (species blocks)
-constexpr int kRedBlocks = 512; // partial sums per reduction
+constexpr int kRedBlocks = 256; // partial sums per reduction and instance
+constexpr int kDenseTM = 64;    // rows per k_zdense workgroup
+constexpr int kDenseHMax = 64;  // k_zdense handles NZ <= 128
 
-// One axis operator applied to every line of one propagator chain at one
-// contour step.
-template <typename T>
+// ------------------------------------------------------------ FFT line op
+
 struct LineOp {
-    const T *src;
-    T *dst;
-    const T *epre;    // Boltzmann factor applied on load (first axis), or null
-    const T *epost;   // Boltzmann factor applied on store (last axis), or null
-    const T *K;       // length L, K[m] of this axis
-    const cpx<T> *tw; // length L, exp(-2 pi i e / L)
+    const void *src;
+    void *dst;
+    const double *epre;  // Boltzmann factor applied on load (first axis), or null
+    const double *epost; // Boltzmann factor applied on store (last axis), or null
+    const double *K;     // length L
+    const cpx<double> *tw;
     int n, L, dirichlet;
-    int nlines;       // real lines
-    int c1, s0, s1;   // line base = (l / c1) * s1 + (l % c1) * s0
-    int stride;       // element stride along the line
+    int nlines;          // real lines
+    int c1, s0, s1;      // line base = (l / c1) * s1 + (l % c1) * s0
+    int stride;          // element stride along the line
     int npass;
     int radix[kMaxPass];
 };
 
-template <typename T>
-__device__ __forceinline__ int line_base(const LineOp<T> &op, int l)
+__device__ __forceinline__ int line_base(const LineOp &op, int l)
 {
     return (l / op.c1) * op.s1 + (l % op.c1) * op.s0;
 }
 
-// grid: (line groups, ops); block: kThreads; dynamic LDS: 2 * P * Lmax * sizeof(cpx<T>)
-template <typename T>
-__global__ void __launch_bounds__(kThreads) k_lineop(const LineOp<T> *__restrict__ ops, int P)
+// grid: (line groups, ops); dynamic LDS: 2 * P * Lmax * 16 bytes
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kThreads) k_lineop(const LineOp *__restrict__ ops, int P)
 {
-    const LineOp<T> &op = ops[blockIdx.y];
+    using T = double;
+    const LineOp &op = ops[blockIdx.y];
     const int lines_per = 2 * P;
     const int g0 = blockIdx.x * lines_per;
     if (g0 >= op.nlines) return;
@@ -59,6 +69,8 @@ __global__ void __launch_bounds__(kThreads) k_lineop(const LineOp<T> *__restrict
     const int nl = min(lines_per, op.nlines - g0);
     const int total = lines_per * n;
     const bool contiguous = (op.stride == 1);
+    const Tin *__restrict__ src = static_cast<const Tin *>(op.src);
+    Tout *__restrict__ dst = static_cast<Tout *>(op.dst);
 
     if (op.dirichlet) {
         for (int c = tid; c < P; c += kThreads) {
@@ -73,7 +85,7 @@ __global__ void __launch_bounds__(kThreads) k_lineop(const LineOp<T> *__restrict
         T v = T(0);
         if (r < nl) {
             const int a = line_base(op, g0 + r) + t * op.stride;
-            v = op.src[a];
+            v = T(src[a]);
             if (op.epre) v *= op.epre[a];
         }
         const int c = r >> 1;
@@ -105,7 +117,105 @@ __global__ void __launch_bounds__(kThreads) k_lineop(const LineOp<T> *__restrict
         const int a = line_base(op, g0 + r) + t * op.stride;
         T v = *slot * scale;
         if (op.epost) v *= op.epost[a];
-        op.dst[a] = v;
+        dst[a] = Tout(v);
+    }
+}
+
+// --------------------------------------------------------- dense z op
+
+struct DenseOp {
+    const void *src;
+    void *dst;
+    const double *epre, *epost;
+    const double *MsT;   // [hs][ldm]: MsT[j][k] = Ms[k][j]
+    const double *MaT;   // [ha][ldm]
+    int n, hs, ha, ldm;
+    int nrows;
+};
+
+// grid: (ceil(rows / kDenseTM), ops); 256 threads = 16 row groups x 16 k groups,
+// each thread 4 rows x 4 k of both halves.
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kThreads) k_zdense(const DenseOp *__restrict__ ops)
+{
+    const DenseOp &op = ops[blockIdx.y];
+    const int row0 = blockIdx.x * kDenseTM;
+    if (row0 >= op.nrows) return;
+    constexpr int LDS = kDenseHMax + 1;   // odd stride: the 4 row groups of a wave hit different banks
+    __shared__ double S[kDenseTM * LDS];
+    __shared__ double Aa[kDenseTM * LDS];
+    const int n = op.n, hs = op.hs, ha = op.ha;
+    const Tin *__restrict__ src = static_cast<const Tin *>(op.src);
+    Tout *__restrict__ dst = static_cast<Tout *>(op.dst);
+    const int tid = threadIdx.x;
+    // symmetric / antisymmetric parts of each row (E applied on load)
+    for (int e = tid; e < kDenseTM * hs; e += kThreads) {
+        const int r = e / hs, j = e - r * hs;
+        const int row = row0 + r;
+        double s = 0.0, a = 0.0;
+        if (row < op.nrows) {
+            const size_t base = (size_t)row * n;
+            double x1 = double(src[base + j]);
+            if (op.epre) x1 *= op.epre[base + j];
+            if (j < ha) {
+                double x2 = double(src[base + n - 1 - j]);
+                if (op.epre) x2 *= op.epre[base + n - 1 - j];
+                s = x1 + x2;
+                a = x1 - x2;
+            } else {
+                s = x1;   // middle point of an odd line
+            }
+        }
+        S[r * LDS + j] = s;
+        if (j < ha) Aa[r * LDS + j] = a;
+    }
+    __syncthreads();
+    const int rg = tid >> 4, kg = tid & 15;
+    for (int k0 = kg * 4; k0 < hs; k0 += 64) {
+        double u[4][4], v[4][4];
+        for (int i = 0; i < 4; ++i)
+            for (int c = 0; c < 4; ++c) { u[i][c] = 0.0; v[i][c] = 0.0; }
+        for (int j = 0; j < hs; ++j) {
+            double s[4];
+            for (int i = 0; i < 4; ++i) s[i] = S[(rg * 4 + i) * LDS + j];
+            const double *m = op.MsT + (size_t)j * op.ldm + k0;
+            const double m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+            for (int i = 0; i < 4; ++i) {
+                u[i][0] += s[i] * m0;
+                u[i][1] += s[i] * m1;
+                u[i][2] += s[i] * m2;
+                u[i][3] += s[i] * m3;
+            }
+        }
+        for (int j = 0; j < ha; ++j) {
+            double a[4];
+            for (int i = 0; i < 4; ++i) a[i] = Aa[(rg * 4 + i) * LDS + j];
+            const double *m = op.MaT + (size_t)j * op.ldm + k0;
+            const double m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+            for (int i = 0; i < 4; ++i) {
+                v[i][0] += a[i] * m0;
+                v[i][1] += a[i] * m1;
+                v[i][2] += a[i] * m2;
+                v[i][3] += a[i] * m3;
+            }
+        }
+        for (int i = 0; i < 4; ++i) {
+            const int row = row0 + rg * 4 + i;
+            if (row >= op.nrows) continue;
+            const size_t base = (size_t)row * n;
+            for (int c = 0; c < 4; ++c) {
+                const int k = k0 + c;
+                if (k >= hs) continue;
+                double y1 = u[i][c] + v[i][c];
+                if (op.epost) y1 *= op.epost[base + k];
+                dst[base + k] = Tout(y1);
+                if (k < ha) {
+                    double y2 = u[i][c] - v[i][c];
+                    if (op.epost) y2 *= op.epost[base + n - 1 - k];
+                    dst[base + n - 1 - k] = Tout(y2);
+                }
+            }
+        }
     }
 }
 
@@ -113,6 +223,8 @@ __global__ void __launch_bounds__(kThreads) k_lineop(const LineOp<T> *__restrict
 
 struct PrepArgs {
     const double *w;     // 13*V
+    double *E;           // 13*V
+    void *q0_dbk1;       // dbk1 q slice 0 (T), or null
     int fields[13];      // fields whose Boltzmann factor is needed
     int nfields;
     double ds;
@@ -122,18 +234,20 @@ struct PrepArgs {
     int nx, ny, nz, delta;
 };
 
+// grid: (blocks, instances)
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_prep(PrepArgs a, T *E, T *q0_dbk1)
+__global__ void __launch_bounds__(kThreads) k_prep(const PrepArgs *__restrict__ args)
 {
+    const PrepArgs &a = args[blockIdx.y];
+    T *q0 = static_cast<T *>(a.q0_dbk1);
     const int stride = gridDim.x * blockDim.x;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < a.V; idx += stride) {
         for (int i = 0; i < a.nfields; ++i) {
-            const int f = a.fields[i];
-            const double w = a.w[(size_t)f * a.V + idx];
-            E[(size_t)f * a.V + idx] = T(exp(a.ds * (-w / 2.0)));
+            const size_t o = (size_t)a.fields[i] * a.V + idx;
+            a.E[o] = exp(a.ds * (-a.w[o] / 2.0));
         }
         if (a.seed_mode == 2) {
-            q0_dbk1[idx] = T(1);
+            q0[idx] = T(1);
         } else if (a.seed_mode == 1) {
             const int k = idx % a.nz;
             const int j = (idx / a.nz) % a.ny;
@@ -141,7 +255,7 @@ __global__ void __launch_bounds__(kThreads) k_prep(PrepArgs a, T *E, T *q0_dbk1)
             const int D = a.delta;
             const bool in = i > (a.nx / 2 - D) && i < (a.nx / 2 + D) && j > (a.ny / 2 - D) &&
                             j < (a.ny / 2 + D) && k > (a.nz / 4 - D) && k < (a.nz / 4 + D);
-            q0_dbk1[idx] = in ? T(1) : T(0);
+            q0[idx] = in ? T(1) : T(0);
         }
     }
 }
@@ -154,7 +268,7 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Block-wide sum of up to 4 values; result valid in thread 0.
+// Block-wide sum of NV values; result valid in thread 0.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV])
 {
@@ -172,89 +286,106 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
         }
 }
 
-// Q partial sums: grid (kRedBlocks, nspecies); ends[y] = q(NS_p - 1) slice.
+// Per-instance scalar block (device).
+struct Scalars {
+    double Q[17];
+    double H;        // F before the ln Q terms
+    double F;
+    double rms;
+    double pmean;
+    double dw_rms;
+    double lnq_frac;
+};
+
+struct QSlot {
+    const void *end;   // q(NS_p - 1) slice (T)
+    double *partial;   // [nblocks]
+    int V;
+};
+
+// grid: (nblocks, slots of all instances)
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_qsum(const T *const *ends, int V, double *partial)
+__global__ void __launch_bounds__(kThreads) k_qsum(const QSlot *__restrict__ slots)
 {
-    const T *q = ends[blockIdx.y];
+    const QSlot &s = slots[blockIdx.y];
+    const T *q = static_cast<const T *>(s.end);
     double v[1] = {0.0};
-    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < V; idx += gridDim.x * kThreads)
+    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < s.V; idx += gridDim.x * kThreads)
         v[0] += double(q[idx]);
     block_sum<1>(v);
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v[0];
+    if (threadIdx.x == 0) s.partial[blockIdx.x] = v[0];
 }
 
 // ------------------------------------------------ densities + F + RMSDev
 
-template <typename T>
 struct DenTerm {
-    const T *q;
-    const T *qd;
+    const void *q;
+    const void *qd;
     int ns, start, stop;
     int field;   // 0..12
-    int qslot;   // index into the Q array
+    int sp;      // species (index into Scalars::Q)
     double frac; // volume factor (vol (1 - coating) or vol_s); <= 0: not applied
 };
 
-template <typename T>
 struct DenArgs {
-    DenTerm<T> term[kMaxTerms];
+    DenTerm term[kMaxTerms];
     int nterms;
     int V;
     int mask;
     int nphi;            // fields carrying density
     int phif[13];
     double chi[13][13];
-    const double *Q;     // per qslot
+    const Scalars *scal;
     const double *w;     // 13*V
     const double *p;     // V
     double *tot;         // 13*V
     double *phitot;      // V
-    double *partial;     // [2][gridDim.x]: H, (phi_tot-1)^2
+    double *partial;     // [2][nblocks]: H, (phi_tot-1)^2
 };
 
 template <typename T>
-__device__ __forceinline__ double simpson(const DenTerm<T> &t, int V, int idx)
+__device__ __forceinline__ double simpson(const DenTerm &t, int V, int idx)
 {
     const size_t sV = (size_t)V;
     const int top = t.ns - 1;
-    auto qc = [&](int s) -> T { return t.q[(size_t)s * sV + idx] * t.qd[(size_t)(top - s) * sV + idx]; };
+    const T *q = static_cast<const T *>(t.q);
+    const T *qd = static_cast<const T *>(t.qd);
+    auto qc = [&](int s) -> double { return double(q[(size_t)s * sV + idx]) * double(qd[(size_t)(top - s) * sV + idx]); };
     if constexpr (sizeof(T) == 8) {
         // the reference's order: ends, then 4*odd, then 2*even (SCFT.c:2283-2297)
-        double phi = double(qc(t.start)) + double(qc(t.stop));
-        for (int s = t.start + 1; s <= t.stop - 1; s += 2) phi = phi + 4.0 * double(qc(s));
-        for (int s = t.start + 2; s <= t.stop - 2; s += 2) phi = phi + 2.0 * double(qc(s));
+        double phi = qc(t.start) + qc(t.stop);
+        for (int s = t.start + 1; s <= t.stop - 1; s += 2) phi = phi + 4.0 * qc(s);
+        for (int s = t.start + 2; s <= t.stop - 2; s += 2) phi = phi + 2.0 * qc(s);
         return phi;
     } else {
-        // FP32 with Kahan-compensated summation
-        float sum = qc(t.start) + qc(t.stop), comp = 0.0f;
+        // FP32 slices: products and the contour sum in FP64 with Kahan compensation
+        double sum = qc(t.start) + qc(t.stop), comp = 0.0;
         for (int s = t.start + 1; s <= t.stop - 1; ++s) {
-            const float wgt = ((s - t.start) & 1) ? 4.0f : 2.0f;
-            const float y = wgt * qc(s) - comp;
-            const float tt = sum + y;
+            const double y = (((s - t.start) & 1) ? 4.0 : 2.0) * qc(s) - comp;
+            const double tt = sum + y;
             comp = (tt - sum) - y;
             sum = tt;
         }
-        return double(sum) - double(comp);
+        return sum;
     }
 }
 
+// grid: (nblocks, instances)
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_density(const DenArgs<T> *__restrict__ argp)
+__global__ void __launch_bounds__(kThreads) k_density(const DenArgs *__restrict__ args)
 {
-    const DenArgs<T> &a = *argp;
+    const DenArgs &a = args[blockIdx.y];
     const int V = a.V;
     double acc[2] = {0.0, 0.0};
     for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < V; idx += gridDim.x * kThreads) {
         double phi[13];
-        for (int f = 0; f < 13; ++f) phi[f] = 0.0;
         bool touched[13];
-        for (int f = 0; f < 13; ++f) touched[f] = false;
+        for (int f = 0; f < 13; ++f) { phi[f] = 0.0; touched[f] = false; }
         for (int i = 0; i < a.nterms; ++i) {
-            const DenTerm<T> &t = a.term[i];
+            const DenTerm &t = a.term[i];
             const double dseff = 1.0 / (t.ns - 1);
             double v = simpson<T>(t, V, idx) / 3.0;
-            v = v * (dseff / a.Q[t.qslot]);
+            v = v * (dseff / a.scal->Q[t.sp]);
             if (t.frac > 0.0) v = t.frac * v;
             phi[t.field] = touched[t.field] ? phi[t.field] + v : v;
             touched[t.field] = true;
@@ -304,13 +435,13 @@ struct UpdArgs {
     double *w;           // 13*V
     double *p;           // V
     const double *tot;   // 13*V
-    double *partial;     // [2][gridDim.x]: sum p_new, sum dw^2
+    double *partial;     // [2][nblocks]: sum p_new, sum dw^2
 };
 
-// funUpdate per voxel (SCFT.c:2047-2113)
-__global__ void __launch_bounds__(kThreads) k_update(const UpdArgs *__restrict__ argp)
+// funUpdate per voxel (SCFT.c:2047-2113); grid: (nblocks, instances)
+__global__ void __launch_bounds__(kThreads) k_update(const UpdArgs *__restrict__ args)
 {
-    const UpdArgs &a = *argp;
+    const UpdArgs &a = args[blockIdx.y];
     const int V = a.V;
     double acc[2] = {0.0, 0.0};
     for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < V; idx += gridDim.x * kThreads) {
@@ -363,16 +494,6 @@ __global__ void __launch_bounds__(kThreads) k_update(const UpdArgs *__restrict__
 
 // ------------------------------------------------------------- finalize
 
-struct Scalars {
-    double Q[17];
-    double H;        // F before the ln Q terms
-    double F;
-    double rms;
-    double pmean;
-    double dw_rms;
-    double lnq_frac;
-};
-
 struct FinArgs {
     int V, nblocks, nq, nact;
     int qsp[17];          // species of each Q slot
@@ -380,24 +501,25 @@ struct FinArgs {
     const double *qpart;  // [nq][nblocks]
     const double *dpart;  // [2][nblocks]
     const double *upart;  // [2][nblocks]
+    double *p;            // V
     Scalars *out;
 };
 
-// Phase 1: Q from its partials (before the density kernel).
-__global__ void k_qfinal(const FinArgs *__restrict__ argp)
+// Q from its partials; grid: (instances), block (kThreads): thread s < nq sums slot s.
+__global__ void __launch_bounds__(kThreads) k_qfinal(const FinArgs *__restrict__ args)
 {
-    const FinArgs &a = *argp;
-    const int s = blockIdx.x;
-    double v[1] = {0.0};
-    for (int b = threadIdx.x; b < a.nblocks; b += kThreads) v[0] += a.qpart[s * a.nblocks + b];
-    block_sum<1>(v);
-    if (threadIdx.x == 0) a.out->Q[a.qsp[s]] = v[0] / a.V;
+    const FinArgs &a = args[blockIdx.x];
+    for (int s = threadIdx.x; s < a.nq; s += kThreads) {
+        double v = 0.0;
+        for (int b = 0; b < a.nblocks; ++b) v += a.qpart[s * a.nblocks + b];
+        a.out->Q[a.qsp[s]] = v / a.V;
+    }
 }
 
-// Phase 2: F, RMSDev, mean p, ||dw||.
-__global__ void k_final(const FinArgs *__restrict__ argp)
+// F, RMSDev, mean p, ||dw||; grid: (instances)
+__global__ void __launch_bounds__(kThreads) k_final(const FinArgs *__restrict__ args)
 {
-    const FinArgs &a = *argp;
+    const FinArgs &a = args[blockIdx.x];
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < a.nblocks; b += kThreads) {
         v[0] += a.dpart[b];
@@ -419,11 +541,15 @@ __global__ void k_final(const FinArgs *__restrict__ argp)
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_pshift(double *p, int V, const Scalars *s)
+// p -= mean(p) (SCFT.c:2119-2123); grid: (blocks, instances)
+__global__ void __launch_bounds__(kThreads) k_pshift(const FinArgs *__restrict__ args)
 {
-    const double m = s->pmean;
-    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < V; idx += gridDim.x * kThreads) p[idx] -= m;
+    const FinArgs &a = args[blockIdx.y];
+    const double m = a.out->pmean;
+    for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < a.V; idx += gridDim.x * kThreads) a.p[idx] -= m;
 }
+
+// ------------------------------------------------------------- utility
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_convert(const double *src, T *dst, int n)

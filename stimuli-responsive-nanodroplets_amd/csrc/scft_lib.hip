@@ -1,25 +1,37 @@
 // scft_lib.hip — libscft: the SCFT.c iteration loop on one MI355X (gfx950).
 //
-// Host side of the C ABI declared in include/scft.h.  One context = one SCFT
-// instance with device-resident fields, propagators (stored [s][idx], one
-// contiguous slice per contour point) and densities, driven on one stream.
-// The iteration is a fixed launch sequence:
-//   k_prep      Boltzmann factors E_f = exp(-DS w_f / 2) (+ dbk1 seed start)
-//   k_lineop    per contour step and axis: every propagator chain of the
-//               iteration advances one step in the same launch
-//   k_qsum / k_qfinal       Q_p = mean q_p(NS_p - 1)            (FP64)
-//   k_density   Simpson contour sums, volume scaling, totals, the free-
-//               energy integrand and (phi_tot - 1)^2 per voxel
-//   k_update    funUpdate per voxel                               (FP64)
-//   k_final / k_pshift      F, RMSDev, mean p, ||dw||; p -= mean(p)
+// Host side of the C ABI declared in include/scft.h.  A context is a batch
+// of SCFT instances ("slots") that share one chain structure: the same
+// species set, NS, MASK, NY, NZ, LY, LZ and DR_TARGET; chi, volumes of the
+// same species set, G1 and LX (hence NX) may differ per slot.  Everything
+// an iteration touches is device-resident: fields, Boltzmann factors,
+// propagators (stored [s][idx], one contiguous slice per contour point),
+// densities and per-slot scalars.  One iteration of every active slot is a
+// fixed launch sequence on the context's stream:
+//   k_prep       E_f = exp(-DS w_f / 2) for every used field (+ dbk1 seed)
+//   per contour step t, per axis (z, y, x): one k_zdense / k_lineop launch
+//                whose ops are every chain of every active slot that still
+//                has a step t (SCFT.c:1092-1456 block structure)
+//   k_qsum, k_qfinal   Q_p = mean q_p(NS_p - 1)                      (FP64)
+//   k_density    Simpson contour sums, volume scaling, totals, free-energy
+//                integrand and (phi_tot - 1)^2
+//   k_update     funUpdate per voxel                                   (FP64)
+//   k_final, k_pshift  F, RMSDev, mean p, ||dw||; p -= mean(p)
+//   one D2H copy of every slot's scalars
+// The sequence is captured once as a hipGraph and replayed while the active
+// set does not change.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/scft.h"
@@ -77,25 +89,39 @@ bool plan_radices(int L, Radices &pl)
 
 // K of one axis on a transform grid of length L (SCFT.c:983-1023 factorised):
 // exp(-alpha k_m^2), k_m = 2 pi (L/2 - |m - L/2|) / L.
-std::vector<double> axis_k(int L, double alpha)
+double axis_k(int m, int L, double alpha)
 {
-    std::vector<double> k(L);
-    for (int m = 0; m < L; ++m) {
-        const double km = 2.0 * M_PI * (L / 2.0 - std::fabs(m - L / 2.0)) / L;
-        k[m] = std::exp(-alpha * std::pow(km, 2));
-    }
-    return k;
+    const double km = 2.0 * M_PI * (L / 2.0 - std::fabs(m - L / 2.0)) / L;
+    return std::exp(-alpha * std::pow(km, 2));
 }
 
-template <typename T>
-std::vector<cpx<T>> twiddles(int L)
+// The z operator as a dense matrix, M[a][b] = response at a to a unit
+// impulse at b, from the eigen-decomposition of the reference's transform:
+//   periodic (length n):  (1/n) sum_m K_m cos(2 pi m (a - b) / n)
+//   Dirichlet (N = 2(n+1)): (2/(n+1)) sum_{m=1..n} K_m sin(pi m (a+1)/(n+1)) sin(pi m (b+1)/(n+1))
+// summed in long double.
+std::vector<double> dense_operator(int n, int dirichlet, double alpha)
 {
-    std::vector<cpx<T>> t(L);
-    for (int e = 0; e < L; ++e) {
-        const double a = 2.0 * M_PI * double(e) / double(L);
-        t[e] = {T(std::cos(a)), T(-std::sin(a))};
-    }
-    return t;
+    std::vector<double> M((size_t)n * n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            long double s = 0.0L;
+            if (dirichlet) {
+                const int N = 2 * (n + 1);
+                for (int m = 1; m <= n; ++m)
+                    s += (long double)axis_k(m, N, alpha) * sinl(pi * m * (a + 1) / (n + 1)) * sinl(pi * m * (b + 1) / (n + 1));
+                s *= 2.0L / (n + 1);
+            } else {
+                for (int m = 0; m < n; ++m) {
+                    const long e = ((long)m * (long)(a - b + n)) % n;
+                    s += (long double)axis_k(m, n, alpha) * cosl(2.0L * pi * e / n);
+                }
+                s /= n;
+            }
+            M[(size_t)a * n + b] = (double)s;
+        }
+    return M;
 }
 
 template <typename X>
@@ -117,469 +143,877 @@ int grid_for(size_t n, int cap = 2048)
     return int(std::max<size_t>(1, std::min<size_t>(g, cap)));
 }
 
-struct Axis {
-    int n, L, dirichlet, nlines, c1, s0, s1, stride;
+// Device-side per-size tables, shared by the slots of a context.
+struct FftTable {
+    int L = 0;
     Radices rad;
-    void *K = nullptr;   // T[L]
-    void *tw = nullptr;  // cpx<T>[L]
+    double *K = nullptr;
+    cpx<double> *tw = nullptr;
+};
+struct DenseTable {
+    int n = 0, hs = 0, ha = 0, ldm = 0;
+    double *MsT = nullptr, *MaT = nullptr;
 };
 
-class EngineBase {
-public:
-    virtual ~EngineBase() = default;
-    virtual int propagate(unsigned long iter) = 0;
-    virtual int densities() = 0;
-    virtual int update(float mf, const double *noise_dev) = 0;
-    virtual int set_mask(const double *mask_dev) = 0;
-    virtual int get_propagator(int sp, int which, int s, double *host) = 0;
-};
+enum ZKind { Z_FFT = 0, Z_DENSE = 1 };
+
+// Process-wide autotuner results: (n, dirichlet, alpha bits, storage) -> ZKind.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, long long, int>, int> g_tune;
 
 }  // namespace
 
-struct scft_ctx {
+// ---------------------------------------------------------------- context
+
+struct Chain {
+    void *base;       // q or q-dagger array of the species (T)
+    double *tmp;      // V doubles between axis operators
+    int nsteps, fa, fb, switch_at;
+};
+
+struct Slot {
     scft_params p{};
     scft_derived d{};
-    int precision = SCFT_FP64;
     int NX = 0, NY = 0, NZ = 0, V = 0;
-    int nred = 1;                 // blocks of each reduction kernel
-    hipStream_t stream = nullptr;
-    double *w = nullptr, *pf = nullptr, *tot = nullptr, *phitot = nullptr, *mask = nullptr;
+    bool ready = false, active = false;
+    unsigned long iter = 0;
+    double *w = nullptr, *pf = nullptr, *tot = nullptr, *phitot = nullptr, *mask = nullptr, *E = nullptr;
+    void *q[SCFT_NSPECIES] = {};
+    void *qd[SCFT_NSPECIES] = {};
     double *qpart = nullptr, *dpart = nullptr, *upart = nullptr;
     double *noise = nullptr;
-    Scalars *scal = nullptr;
-    Scalars *scal_host = nullptr; // pinned
-    FinArgs *fin = nullptr;
-    UpdArgs upd_host{};
-    UpdArgs *upd = nullptr;
-    std::unique_ptr<EngineBase> eng;
+    std::vector<Chain> chains;
+    const FftTable *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    const DenseTable *dz = nullptr;
+    PrepArgs prep{};
+    DenArgs den{};
+    UpdArgs upd{};
+    FinArgs fin{};
+    std::vector<QSlot> qslots;
+};
+
+struct Launch {
+    int kind;        // 0 lineop, 1 dense
+    int pos;         // 0 first, 1 middle, 2 last, 3 only
+    int first, count, P, gx;
+    size_t lds;
+};
+
+struct scft_ctx {
+    int precision = SCFT_FP64;
+    size_t tsize = 8;                // sizeof(T)
+    scft_params proto{};
+    scft_derived dproto{};
+    int nslots = 0, max_nx = 0, max_V = 0;
+    int NY = 0, NZ = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Slot> slots;
+    std::vector<void *> allocs;
+    std::vector<std::unique_ptr<FftTable>> ffts;
+    std::vector<std::unique_ptr<DenseTable>> denses;
+    int zkind = Z_FFT;
+    Scalars *scal = nullptr;         // [nslots] device
+    Scalars *scal_host = nullptr;    // [nslots] pinned
+    // schedule of the active set
+    bool dirty = true;
+    std::vector<int> act;            // active slot ids
+    std::vector<Launch> launches;
+    LineOp *lineops = nullptr;
+    size_t lineops_cap = 0;
+    DenseOp *denseops = nullptr;
+    size_t denseops_cap = 0;
+    PrepArgs *d_prep = nullptr;
+    DenArgs *d_den = nullptr;
+    UpdArgs *d_upd = nullptr;
+    FinArgs *d_fin = nullptr;
+    QSlot *d_qslots = nullptr;
+    size_t d_qslots_cap = 0;
+    int nqslots = 0;
+    int nred = 1;
+    // graph of one iteration of the active set
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
 
-template <typename T>
-class Engine final : public EngineBase {
-public:
-    explicit Engine(scft_ctx *c) : c_(c) {}
-    ~Engine() override
-    {
-        for (void *ptr : allocs_) (void)hipFree(ptr);
+template <typename X>
+int ctx_alloc(scft_ctx *c, X **p, size_t n)
+{
+    RETCHK(dev_alloc(p, n));
+    c->allocs.push_back(*p);
+    return SCFT_OK;
+}
+
+void fill_d(scft_ctx *c, double *p, double v, size_t n)
+{
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, p, v, n);
+}
+
+void fill_t(scft_ctx *c, void *p, double v, size_t n)
+{
+    if (c->tsize == 8)
+        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, (double *)p, v, n);
+    else
+        hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kThreads), 0, c->stream, (float *)p, (float)v, n);
+}
+
+// Slots must share everything that shapes the launch schedule.
+bool same_structure(const scft_params &a, const scft_params &b)
+{
+    if (a.ns != b.ns || a.mask != b.mask || a.ly != b.ly || a.lz != b.lz || a.dr_target != b.dr_target) return false;
+    if ((a.lx == 1.0) != (b.lx == 1.0)) return false;   // every slot has an x axis, or none has
+    for (int i = 0; i < 4; ++i) {
+        if ((a.vol_dbk[i] > 0) != (b.vol_dbk[i] > 0)) return false;
+        if (a.vol_dbk[i] > 0 && (a.len_dbk[i] != b.len_dbk[i] || a.f_dbk[i] != b.f_dbk[i])) return false;
+    }
+    for (int i = 0; i < 10; ++i) {
+        if ((a.vol_homo[i] > 0) != (b.vol_homo[i] > 0)) return false;
+        if (a.vol_homo[i] > 0 && a.len_homo[i] != b.len_homo[i]) return false;
+    }
+    for (int i = 0; i < 3; ++i) {
+        if ((a.vol_s[i] > 0) != (b.vol_s[i] > 0)) return false;
+        if (a.vol_s[i] > 0 && a.len_s[i] != b.len_s[i]) return false;
+    }
+    return true;
+}
+
+int get_fft(scft_ctx *c, int L, double alpha, const FftTable **out)
+{
+    auto t = std::make_unique<FftTable>();
+    t->L = L;
+    if (!plan_radices(L, t->rad)) {
+        set_err("transform length " + std::to_string(L) + " has too many factors");
+        return SCFT_EUNSUPPORTED;
+    }
+    if (size_t(2) * L * sizeof(cpx<double>) > 160 * 1024) {
+        set_err("transform length " + std::to_string(L) + " exceeds the LDS tile");
+        return SCFT_EUNSUPPORTED;
+    }
+    std::vector<double> K(L);
+    std::vector<cpx<double>> tw(L);
+    for (int m = 0; m < L; ++m) {
+        K[m] = axis_k(m, L, alpha);
+        const double a = 2.0 * M_PI * double(m) / double(L);
+        tw[m] = {std::cos(a), -std::sin(a)};
+    }
+    RETCHK(ctx_alloc(c, &t->K, L));
+    RETCHK(ctx_alloc(c, &t->tw, L));
+    HIPCHK(hipMemcpy(t->K, K.data(), L * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->tw, tw.data(), L * sizeof(cpx<double>), hipMemcpyHostToDevice));
+    *out = t.get();
+    c->ffts.push_back(std::move(t));
+    return SCFT_OK;
+}
+
+int get_dense(scft_ctx *c, int n, int dirichlet, double alpha, const DenseTable **out)
+{
+    auto t = std::make_unique<DenseTable>();
+    t->n = n;
+    t->ha = n / 2;
+    t->hs = n - t->ha;
+    t->ldm = ((t->hs + 3) / 4) * 4;
+    std::vector<double> M = dense_operator(n, dirichlet, alpha);
+    std::vector<double> MsT((size_t)t->hs * t->ldm, 0.0), MaT((size_t)std::max(1, t->ha) * t->ldm, 0.0);
+    // Ms[k][j] = (M[k][j] + M[k][n-1-j]) / 2 (j < ha), Ms[k][mid] = M[k][mid];
+    // Ma[k][j] = (M[k][j] - M[k][n-1-j]) / 2; stored transposed.
+    for (int k = 0; k < t->hs; ++k) {
+        for (int j = 0; j < t->ha; ++j) {
+            MsT[(size_t)j * t->ldm + k] = 0.5 * (M[(size_t)k * n + j] + M[(size_t)k * n + n - 1 - j]);
+            MaT[(size_t)j * t->ldm + k] = 0.5 * (M[(size_t)k * n + j] - M[(size_t)k * n + n - 1 - j]);
+        }
+        if (t->hs > t->ha) MsT[(size_t)t->ha * t->ldm + k] = M[(size_t)k * n + t->ha];
+    }
+    RETCHK(ctx_alloc(c, &t->MsT, MsT.size()));
+    RETCHK(ctx_alloc(c, &t->MaT, MaT.size()));
+    HIPCHK(hipMemcpy(t->MsT, MsT.data(), MsT.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->MaT, MaT.data(), MaT.size() * sizeof(double), hipMemcpyHostToDevice));
+    *out = t.get();
+    c->denses.push_back(std::move(t));
+    return SCFT_OK;
+}
+
+int lines_P(int L)
+{
+    const int budget = 64 * 1024;
+    int P = budget / int(2 * L * sizeof(cpx<double>));
+    return std::max(1, std::min(P, 16));
+}
+
+// ------------------------------------------------------------- slot setup
+
+int slot_alloc(scft_ctx *c, Slot &s)
+{
+    const size_t V = c->max_V;
+    const scft_derived &d = c->dproto;
+    RETCHK(ctx_alloc(c, &s.w, 13 * V));
+    RETCHK(ctx_alloc(c, &s.pf, V));
+    RETCHK(ctx_alloc(c, &s.tot, 13 * V));
+    RETCHK(ctx_alloc(c, &s.phitot, V));
+    RETCHK(ctx_alloc(c, &s.mask, 3 * V));
+    RETCHK(ctx_alloc(c, &s.E, 13 * V));
+    for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
+        const int ns = d.ns_sp[sp];
+        if (ns <= 0) continue;
+        char *q = nullptr;
+        RETCHK(ctx_alloc(c, &q, (size_t)ns * V * c->tsize));
+        s.q[sp] = q;
+        if (sp >= 4 && sp < 14) {
+            s.qd[sp] = q;   // q-dagger = q for homopolymers (SCFT.c:1102)
+        } else {
+            char *qd = nullptr;
+            RETCHK(ctx_alloc(c, &qd, (size_t)ns * V * c->tsize));
+            s.qd[sp] = qd;
+        }
+    }
+    RETCHK(ctx_alloc(c, &s.qpart, (size_t)SCFT_NSPECIES * kRedBlocks));
+    RETCHK(ctx_alloc(c, &s.dpart, 2 * (size_t)kRedBlocks));
+    RETCHK(ctx_alloc(c, &s.upart, 2 * (size_t)kRedBlocks));
+    // chains in the reference's species order (SCFT.c:1092-1456)
+    s.chains.clear();
+    auto add = [&](void *base, int nsteps, int fa, int fb, int sw) {
+        s.chains.push_back(Chain{base, nullptr, nsteps, fa, fb, sw});
+    };
+    for (int h = 0; h < 10; ++h)
+        if (d.ns_sp[4 + h] > 0) add(s.q[4 + h], d.ns_sp[4 + h] - 1, h, h, 1 << 30);
+    for (int b = 0; b < 4; ++b) {
+        if (d.ns_sp[b] <= 0) continue;
+        const int fa = kDbkField[b][0], fb = kDbkField[b][1];
+        add(s.q[b], d.ns_sp[b] - 1, fa, fb, d.ns_x[b] - 1);   // A then B (SCFT.c:1250-1256)
+        add(s.qd[b], d.ns_sp[b] - 1, fb, fa, d.ns_y[b]);      // B then A (SCFT.c:1263-1269)
+    }
+    for (int b = 0; b < 3; ++b) {
+        const int sp = 14 + b;
+        if (d.ns_sp[sp] <= 0) continue;
+        add(s.q[sp], d.ns_sp[sp] - 1, 10 + b, 10 + b, 1 << 30);   // from the mask (SCFT.c:2441)
+        add(s.qd[sp], d.ns_sp[sp] - 1, 10 + b, 10 + b, 1 << 30);
+    }
+    for (auto &ch : s.chains) RETCHK(ctx_alloc(c, &ch.tmp, V));
+    return SCFT_OK;
+}
+
+// (Re)initialise a slot for parameter set p: tables, argument blocks, zero
+// state, q[0] = 1 (SCFT.c:785-843, 881-981).
+int slot_init(scft_ctx *c, Slot &s, const scft_params &p)
+{
+    scft_derived d;
+    int rc = scft_derive(&p, &d);
+    if (rc != SCFT_OK) {
+        set_err(rc == SCFT_EVOLUME ? "volumes do not sum to 1" : "invalid parameters");
+        return rc;
+    }
+    if (!same_structure(p, c->proto) || d.ny != c->NY || d.nz != c->NZ) {
+        set_err("slot parameters do not share the batch's chain structure");
+        return SCFT_EINVAL;
+    }
+    if (d.nx > c->max_nx) {
+        set_err("slot NX exceeds the batch capacity");
+        return SCFT_EINVAL;
+    }
+    s.p = p;
+    s.d = d;
+    s.NX = d.nx;
+    s.NY = d.ny;
+    s.NZ = d.nz;
+    s.V = d.nx * d.ny * d.nz;
+    const size_t V = s.V;
+    const double DS = 1.0 / (p.ns - 1);
+    const double DX = p.lx / s.NX, DY = p.ly / s.NY, DZ = p.lz / s.NZ;
+
+    // transform tables (cached per context)
+    auto find_fft = [&](int L, double alpha, const FftTable **out) -> int {
+        for (auto &f : c->ffts) {
+            if (f->L != L) continue;
+            // same K? check one entry on the host copy by recomputation
+            double kh = axis_k(1 % L, L, alpha), kd = 0.0;
+            HIPCHK(hipMemcpy(&kd, f->K + (1 % L), sizeof(double), hipMemcpyDeviceToHost));
+            if (kh == kd) { *out = f.get(); return SCFT_OK; }
+        }
+        return get_fft(c, L, alpha, out);
+    };
+    const int dir = p.mask ? 1 : 0;
+    s.fz = nullptr;
+    s.dz = nullptr;
+    const double az = DS / (DZ * DZ);
+    if (c->zkind == Z_DENSE) {
+        for (auto &t : c->denses)
+            if (t->n == s.NZ) s.dz = t.get();
+        if (!s.dz) RETCHK(get_dense(c, s.NZ, dir, az, &s.dz));
+    } else {
+        RETCHK(find_fft(dir ? 2 * (s.NZ + 1) : s.NZ, az, &s.fz));
+    }
+    s.fy = nullptr;
+    if (s.NY > 1) RETCHK(find_fft(s.NY, DS / (DY * DY), &s.fy));
+    s.fx = nullptr;
+    if (s.NX > 1) RETCHK(find_fft(s.NX, DS / (DX * DX), &s.fx));
+
+    // zero state
+    fill_d(c, s.w, 0.0, 13 * V);
+    fill_d(c, s.pf, 0.0, V);
+    fill_d(c, s.tot, 0.0, 13 * V);
+    fill_d(c, s.phitot, 0.0, V);
+    fill_d(c, s.mask, 0.0, 3 * V);
+    for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
+        if (d.ns_sp[sp] <= 0) continue;
+        fill_t(c, s.q[sp], 1.0, V);
+        if (s.qd[sp] != s.q[sp]) fill_t(c, s.qd[sp], 1.0, V);
     }
 
-    int init()
-    {
-        scft_ctx &c = *c_;
-        const scft_derived &d = c.d;
-        const size_t V = c.V;
-        // ---- propagator storage and chains
-        RETCHK(alloc(&E_, 13 * V));
-        for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
-            const int ns = d.ns_sp[sp];
-            if (ns <= 0) continue;
-            RETCHK(alloc(&q_[sp], (size_t)ns * V));
-            fill(q_[sp], T(1), V);                       // q[0] = 1 (SCFT.c:881-981)
-            const bool homo = sp >= 4 && sp < 14;
-            if (!homo) {
-                RETCHK(alloc(&qd_[sp], (size_t)ns * V));
-                fill(qd_[sp], T(1), V);
-            } else {
-                qd_[sp] = q_[sp];                        // q-dagger = q for homopolymers
-            }
-        }
-        // chains in the reference's species order (SCFT.c:1092-1456)
-        for (int h = 0; h < 10; ++h)
-            if (d.ns_sp[4 + h] > 0) add_chain(q_[4 + h], d.ns_sp[4 + h] - 1, h, h, 1 << 30);
-        for (int b = 0; b < 4; ++b) {
-            if (d.ns_sp[b] <= 0) continue;
-            const int fa = kDbkField[b][0], fb = kDbkField[b][1];
-            add_chain(q_[b], d.ns_sp[b] - 1, fa, fb, d.ns_x[b] - 1);   // A then B (1250-1256)
-            add_chain(qd_[b], d.ns_sp[b] - 1, fb, fa, d.ns_y[b]);      // B then A (1263-1269)
-        }
-        for (int s = 0; s < 3; ++s) {
-            const int sp = 14 + s;
-            if (d.ns_sp[sp] <= 0) continue;
-            add_chain(q_[sp], d.ns_sp[sp] - 1, 10 + s, 10 + s, 1 << 30);  // mask start (2441)
-            add_chain(qd_[sp], d.ns_sp[sp] - 1, 10 + s, 10 + s, 1 << 30);
-        }
-        for (auto &ch : chains_) RETCHK(alloc(&ch.tmp, V));
+    // prep
+    PrepArgs &pa = s.prep;
+    std::memset(&pa, 0, sizeof(pa));
+    bool used[13] = {false};
+    for (auto &ch : s.chains) used[ch.fa] = used[ch.fb] = true;
+    for (int f = 0; f < 13; ++f)
+        if (used[f]) pa.fields[pa.nfields++] = f;
+    pa.w = s.w;
+    pa.E = s.E;
+    pa.q0_dbk1 = d.ns_sp[0] > 0 ? s.q[0] : nullptr;
+    pa.ds = DS;
+    pa.V = s.V;
+    pa.nx = s.NX;
+    pa.ny = s.NY;
+    pa.nz = s.NZ;
+    pa.delta = p.seed_area_delta;
 
-        // ---- axes
-        const double DS = 1.0 / (c.p.ns - 1);
-        const double DX = c.p.lx / c.NX, DY = c.p.ly / c.NY, DZ = c.p.lz / c.NZ;
-        axes_.clear();
-        {   // z (contiguous lines), always applied
-            Axis a{};
-            a.n = c.NZ;
-            a.dirichlet = c.p.mask ? 1 : 0;
-            a.L = a.dirichlet ? 2 * (c.NZ + 1) : c.NZ;
-            a.nlines = c.NX * c.NY;
-            a.c1 = 1; a.s0 = 0; a.s1 = c.NZ; a.stride = 1;
-            RETCHK(setup_axis(a, DS / (DZ * DZ)));
-            axes_.push_back(a);
-        }
-        if (c.NY > 1) {
-            Axis a{};
-            a.n = a.L = c.NY;
-            a.nlines = c.NX * c.NZ;
-            a.c1 = c.NZ; a.s0 = 1; a.s1 = c.NZ * c.NY; a.stride = c.NZ;
-            RETCHK(setup_axis(a, DS / (DY * DY)));
-            axes_.push_back(a);
-        }
-        if (c.NX > 1) {
-            Axis a{};
-            a.n = a.L = c.NX;
-            a.nlines = c.NY * c.NZ;
-            a.c1 = c.NY * c.NZ; a.s0 = 1; a.s1 = 0; a.stride = c.NY * c.NZ;
-            RETCHK(setup_axis(a, DS / (DX * DX)));
-            axes_.push_back(a);
-        }
+    // Q slots: dbk1/2 from q-dagger, dbk3/4 and the rest from q
+    // (SCFT.c:1271, 1314, 1357, 1399, 1422); ln Q coefficients (SCFT.c:1529-1555)
+    FinArgs &fin = s.fin;
+    std::memset(&fin, 0, sizeof(fin));
+    s.qslots.clear();
+    for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
+        const int ns = d.ns_sp[sp];
+        if (ns <= 0) continue;
+        const void *src = (sp < 2) ? s.qd[sp] : s.q[sp];
+        QSlot qs;
+        qs.end = static_cast<const char *>(src) + (size_t)(ns - 1) * V * c->tsize;
+        qs.partial = s.qpart + (size_t)fin.nq * c->nred;   // k_qfinal reads [nq][nred]
+        qs.V = s.V;
+        s.qslots.push_back(qs);
+        fin.qsp[fin.nq] = sp;
+        double coef;
+        if (sp < 4) coef = p.vol_dbk[sp] / d.alpha[sp] * (1.0 - d.coating_vol);
+        else if (sp < 14) coef = p.vol_homo[sp - 4] / d.alpha[sp] * (1.0 - d.coating_vol);
+        else coef = p.mask ? p.vol_s[sp - 14] / d.alpha[sp] : 0.0;
+        fin.lnq_coef[fin.nq] = coef;
+        fin.nq++;
+    }
+    fin.V = s.V;
+    fin.nblocks = c->nred;
+    fin.nact = d.n_active;
+    fin.qpart = s.qpart;
+    fin.dpart = s.dpart;
+    fin.upart = s.upart;
+    fin.p = s.pf;
+    fin.out = c->scal + (&s - c->slots.data());
 
-        // ---- step schedule: for each step t, for each axis, one op per live chain
-        maxsteps_ = 0;
-        for (auto &ch : chains_) maxsteps_ = std::max(maxsteps_, ch.nsteps);
-        std::vector<LineOp<T>> ops;
-        launches_.clear();
-        const int nax = int(axes_.size());
-        for (int t = 0; t < maxsteps_; ++t) {
-            for (int ai = 0; ai < nax; ++ai) {
-                const Axis &ax = axes_[ai];
-                Launch ln{};
-                ln.first = int(ops.size());
-                int maxl = 0;
-                for (auto &ch : chains_) {
+    // density terms (SCFT.c:1092-1513)
+    DenArgs &den = s.den;
+    std::memset(&den, 0, sizeof(den));
+    auto term = [&](int sp, int start, int stop, int field, double frac) {
+        DenTerm &t = den.term[den.nterms++];
+        t.q = s.q[sp];
+        t.qd = s.qd[sp];
+        t.ns = d.ns_sp[sp];
+        t.start = start;
+        t.stop = stop;
+        t.field = field;
+        t.sp = sp;
+        t.frac = frac;
+    };
+    const double cv = d.coating_vol;
+    for (int h = 0; h < 10; ++h)
+        if (d.ns_sp[4 + h] > 0) term(4 + h, 0, d.ns_sp[4 + h] - 1, h, p.vol_homo[h] * (1.0 - cv));
+    for (int b = 0; b < 4; ++b) {
+        if (d.ns_sp[b] <= 0) continue;
+        const double fr = p.vol_dbk[b] * (1.0 - cv);
+        term(b, 0, d.ns_x[b] - 1, kDbkField[b][0], fr);
+        term(b, d.ns_x[b] - 1, d.ns_sp[b] - 1, kDbkField[b][1], fr);
+    }
+    for (int b = 0; b < 3; ++b)
+        if (d.ns_sp[14 + b] > 0) term(14 + b, 0, d.ns_sp[14 + b] - 1, 10 + b, p.mask ? p.vol_s[b] : 0.0);
+    bool carries[13] = {false};
+    for (int i = 0; i < den.nterms; ++i) carries[den.term[i].field] = true;
+    for (int f = 0; f < 13; ++f)
+        if (carries[f]) den.phif[den.nphi++] = f;
+    for (int f = 0; f < 13; ++f)
+        for (int g = 0; g < 13; ++g) den.chi[f][g] = d.chi_eff[f][g];
+    den.V = s.V;
+    den.mask = p.mask;
+    den.scal = fin.out;
+    den.w = s.w;
+    den.p = s.pf;
+    den.tot = s.tot;
+    den.phitot = s.phitot;
+    den.partial = s.dpart;
+
+    // update (SCFT.c:1993-2124)
+    UpdArgs &u = s.upd;
+    std::memset(&u, 0, sizeof(u));
+    u.V = s.V;
+    u.nx = s.NX;
+    u.ny = s.NY;
+    u.nz = s.NZ;
+    for (int f = 0; f < 13; ++f)
+        if (d.active[f]) u.act[u.nact++] = f;
+    for (int f = 0; f < 13; ++f) {
+        u.phibar[f] = d.phi_avg[f];
+        for (int g = 0; g < 13; ++g) u.chi[f][g] = d.chi_eff[f][g];
+    }
+    u.g1 = p.g1;
+    u.g2 = p.g1 / 6.0;
+    u.w = s.w;
+    u.p = s.pf;
+    u.tot = s.tot;
+    u.partial = s.upart;
+    s.ready = true;
+    c->dirty = true;
+    return SCFT_OK;
+}
+
+// ------------------------------------------------------------- schedule
+
+template <typename X>
+int upload(scft_ctx *c, X **dev, size_t *cap, const std::vector<X> &v)
+{
+    if (v.size() > *cap) {
+        if (*dev) {
+            auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)*dev);
+            if (it != c->allocs.end()) c->allocs.erase(it);
+            (void)hipFree(*dev);
+        }
+        size_t n = std::max<size_t>(v.size(), 2 * *cap);
+        RETCHK(ctx_alloc(c, dev, n));
+        *cap = n;
+    }
+    if (!v.empty())
+        HIPCHK(hipMemcpyAsync(*dev, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, c->stream));
+    return SCFT_OK;
+}
+
+void drop_graph(scft_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+int build_schedule(scft_ctx *c)
+{
+    drop_graph(c);
+    c->act.clear();
+    for (int i = 0; i < c->nslots; ++i)
+        if (c->slots[i].ready && c->slots[i].active) c->act.push_back(i);
+    c->launches.clear();
+    std::vector<LineOp> lops;
+    std::vector<DenseOp> dops;
+    int maxsteps = 0;
+    for (int si : c->act)
+        for (auto &ch : c->slots[si].chains) maxsteps = std::max(maxsteps, ch.nsteps);
+    const size_t ts = c->tsize;
+    for (int t = 0; t < maxsteps; ++t) {
+        // axes in order z, y, x; every slot of the batch shares NY/NZ, so the
+        // axis list is the same for all (x exists when any slot has NX > 1)
+        std::vector<int> axes = {2};
+        if (c->NY > 1) axes.push_back(1);
+        bool anyx = false;
+        for (int si : c->act) anyx |= c->slots[si].NX > 1;
+        if (anyx) axes.push_back(0);
+        const int nax = int(axes.size());
+        for (int ai = 0; ai < nax; ++ai) {
+            const int axis = axes[ai];
+            const int pos = (nax == 1) ? 3 : (ai == 0 ? 0 : (ai == nax - 1 ? 2 : 1));
+            const bool dense = (axis == 2 && c->zkind == Z_DENSE);
+            Launch ln{};
+            ln.kind = dense ? 1 : 0;
+            ln.pos = pos;
+            ln.first = dense ? int(dops.size()) : int(lops.size());
+            int maxl = 0, maxL = 1;
+            for (int si : c->act) {
+                Slot &s = c->slots[si];
+                const size_t V = s.V;
+                for (auto &ch : s.chains) {
                     if (ch.nsteps <= t) continue;
                     const int f = (t < ch.switch_at) ? ch.fa : ch.fb;
-                    LineOp<T> op{};
-                    op.src = (ai == 0) ? ch.base + (size_t)t * V : ch.tmp;
-                    op.dst = (ai == nax - 1) ? ch.base + (size_t)(t + 1) * V : ch.tmp;
-                    op.epre = (ai == 0) ? E_ + (size_t)f * V : nullptr;
-                    op.epost = (ai == nax - 1) ? E_ + (size_t)f * V : nullptr;
-                    op.K = static_cast<const T *>(ax.K);
-                    op.tw = static_cast<const cpx<T> *>(ax.tw);
-                    op.n = ax.n; op.L = ax.L; op.dirichlet = ax.dirichlet;
-                    op.nlines = ax.nlines;
-                    op.c1 = ax.c1; op.s0 = ax.s0; op.s1 = ax.s1; op.stride = ax.stride;
-                    op.npass = ax.rad.npass;
-                    for (int i = 0; i < kMaxPass; ++i) op.radix[i] = i < ax.rad.npass ? ax.rad.r[i] : 1;
-                    ops.push_back(op);
-                    maxl = std::max(maxl, ax.nlines);
+                    const void *src = (ai == 0) ? static_cast<const char *>(ch.base) + (size_t)t * V * ts : (const void *)ch.tmp;
+                    void *dst = (ai == nax - 1) ? static_cast<char *>(ch.base) + (size_t)(t + 1) * V * ts : (void *)ch.tmp;
+                    const double *ep = s.E + (size_t)f * V;
+                    if (dense) {
+                        DenseOp op{};
+                        op.src = src;
+                        op.dst = dst;
+                        op.epre = (ai == 0) ? ep : nullptr;
+                        op.epost = (ai == nax - 1) ? ep : nullptr;
+                        op.MsT = s.dz->MsT;
+                        op.MaT = s.dz->MaT;
+                        op.n = s.dz->n;
+                        op.hs = s.dz->hs;
+                        op.ha = s.dz->ha;
+                        op.ldm = s.dz->ldm;
+                        op.nrows = s.NX * s.NY;
+                        dops.push_back(op);
+                        maxl = std::max(maxl, op.nrows);
+                        continue;
+                    }
+                    const FftTable *ft = axis == 2 ? s.fz : (axis == 1 ? s.fy : s.fx);
+                    LineOp op{};
+                    op.src = src;
+                    op.dst = dst;
+                    op.epre = (ai == 0) ? ep : nullptr;
+                    op.epost = (ai == nax - 1) ? ep : nullptr;
+                    op.K = ft->K;
+                    op.tw = ft->tw;
+                    op.L = ft->L;
+                    op.npass = ft->rad.npass;
+                    for (int i = 0; i < kMaxPass; ++i) op.radix[i] = i < ft->rad.npass ? ft->rad.r[i] : 1;
+                    if (axis == 2) {
+                        op.n = s.NZ;
+                        op.dirichlet = s.p.mask ? 1 : 0;
+                        op.nlines = s.NX * s.NY;
+                        op.c1 = 1; op.s0 = 0; op.s1 = s.NZ; op.stride = 1;
+                    } else if (axis == 1) {
+                        op.n = s.NY;
+                        op.nlines = s.NX * s.NZ;
+                        op.c1 = s.NZ; op.s0 = 1; op.s1 = s.NZ * s.NY; op.stride = s.NZ;
+                    } else {
+                        op.n = s.NX;
+                        op.nlines = s.NY * s.NZ;
+                        op.c1 = s.NY * s.NZ; op.s0 = 1; op.s1 = 0; op.stride = s.NY * s.NZ;
+                    }
+                    lops.push_back(op);
+                    maxl = std::max(maxl, op.nlines);
+                    maxL = std::max(maxL, op.L);
                 }
-                ln.count = int(ops.size()) - ln.first;
-                ln.P = lines_P(ax.L);
-                ln.lds = size_t(2) * ln.P * ax.L * sizeof(cpx<T>);
+            }
+            ln.count = (dense ? int(dops.size()) : int(lops.size())) - ln.first;
+            if (ln.count == 0) continue;
+            if (dense) {
+                ln.gx = (maxl + kDenseTM - 1) / kDenseTM;
+            } else {
+                ln.P = lines_P(maxL);
+                ln.lds = size_t(2) * ln.P * maxL * sizeof(cpx<double>);
                 ln.gx = (maxl + 2 * ln.P - 1) / (2 * ln.P);
-                if (ln.count > 0) launches_.push_back(ln);
+            }
+            c->launches.push_back(ln);
+        }
+    }
+    RETCHK(upload(c, &c->lineops, &c->lineops_cap, lops));
+    RETCHK(upload(c, &c->denseops, &c->denseops_cap, dops));
+    // per-slot argument arrays in active order
+    std::vector<PrepArgs> prep;
+    std::vector<DenArgs> den;
+    std::vector<UpdArgs> upd;
+    std::vector<FinArgs> fin;
+    std::vector<QSlot> qs;
+    for (int si : c->act) {
+        Slot &s = c->slots[si];
+        prep.push_back(s.prep);
+        den.push_back(s.den);
+        upd.push_back(s.upd);
+        fin.push_back(s.fin);
+        for (auto &q : s.qslots) qs.push_back(q);
+    }
+    size_t cap = c->slots.size();
+    if (!c->d_prep) {
+        RETCHK(ctx_alloc(c, &c->d_prep, cap));
+        RETCHK(ctx_alloc(c, &c->d_den, cap));
+        RETCHK(ctx_alloc(c, &c->d_upd, cap));
+        RETCHK(ctx_alloc(c, &c->d_fin, cap));
+    }
+    if (!prep.empty()) {
+        HIPCHK(hipMemcpyAsync(c->d_prep, prep.data(), prep.size() * sizeof(PrepArgs), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_den, den.data(), den.size() * sizeof(DenArgs), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_upd, upd.data(), upd.size() * sizeof(UpdArgs), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_fin, fin.data(), fin.size() * sizeof(FinArgs), hipMemcpyHostToDevice, c->stream));
+    }
+    RETCHK(upload(c, &c->d_qslots, &c->d_qslots_cap, qs));
+    c->nqslots = int(qs.size());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->dirty = false;
+    return SCFT_OK;
+}
+
+// Per-iteration argument changes (seed mode, noise) go to the device
+// argument blocks; the launch sequence itself does not change.
+int set_iteration_args(scft_ctx *c, float mf, const double *noise_dev)
+{
+    bool changed = false;
+    for (size_t a = 0; a < c->act.size(); ++a) {
+        Slot &s = c->slots[c->act[a]];
+        int mode = 0;
+        if (s.d.ns_sp[0] > 0 && s.p.enable_seed) mode = (s.iter <= (unsigned long)s.p.seed_steps) ? 1 : 2;
+        if (mode != s.prep.seed_mode) {
+            s.prep.seed_mode = mode;
+            HIPCHK(hipMemcpyAsync(c->d_prep + a, &s.prep, sizeof(PrepArgs), hipMemcpyHostToDevice, c->stream));
+            changed = true;
+        }
+        const double amp = s.p.dt_noise * mf;
+        const double *nz = (mf != 0.0f) ? noise_dev : nullptr;
+        if (amp != s.upd.amp || nz != s.upd.noise) {
+            s.upd.amp = amp;
+            s.upd.noise = nz;
+            HIPCHK(hipMemcpyAsync(c->d_upd + a, &s.upd, sizeof(UpdArgs), hipMemcpyHostToDevice, c->stream));
+            changed = true;
+        }
+    }
+    (void)changed;
+    return SCFT_OK;
+}
+
+template <typename T>
+void launch_prep(scft_ctx *c, int nact, int maxV)
+{
+    hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(maxV, 512), nact), dim3(kThreads), 0, c->stream,
+                       (const PrepArgs *)c->d_prep);
+}
+
+template <typename T>
+void launch_steps(scft_ctx *c)
+{
+    for (const Launch &ln : c->launches) {
+        if (ln.kind == 1) {
+            const DenseOp *ops = c->denseops + ln.first;
+            dim3 g(ln.gx, ln.count);
+            switch (ln.pos) {
+            case 0: hipLaunchKernelGGL((k_zdense<T, double>), g, dim3(kThreads), 0, c->stream, ops); break;
+            case 1: hipLaunchKernelGGL((k_zdense<double, double>), g, dim3(kThreads), 0, c->stream, ops); break;
+            case 2: hipLaunchKernelGGL((k_zdense<double, T>), g, dim3(kThreads), 0, c->stream, ops); break;
+            default: hipLaunchKernelGGL((k_zdense<T, T>), g, dim3(kThreads), 0, c->stream, ops); break;
+            }
+        } else {
+            const LineOp *ops = c->lineops + ln.first;
+            dim3 g(ln.gx, ln.count);
+            switch (ln.pos) {
+            case 0: hipLaunchKernelGGL((k_lineop<T, double>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
+            case 1: hipLaunchKernelGGL((k_lineop<double, double>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
+            case 2: hipLaunchKernelGGL((k_lineop<double, T>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
+            default: hipLaunchKernelGGL((k_lineop<T, T>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
             }
         }
-        RETCHK(alloc(&ops_dev_, std::max<size_t>(1, ops.size())));
-        HIPCHK(hipMemcpy(ops_dev_, ops.data(), ops.size() * sizeof(LineOp<T>), hipMemcpyHostToDevice));
+    }
+}
 
-        // ---- prep: fields whose Boltzmann factor the chains use
-        std::memset(&prep_, 0, sizeof(prep_));
-        bool used[13] = {false};
-        for (auto &ch : chains_) used[ch.fa] = used[ch.fb] = true;
-        for (int f = 0; f < 13; ++f)
-            if (used[f]) prep_.fields[prep_.nfields++] = f;
-        prep_.w = c.w;
-        prep_.ds = DS;
-        prep_.V = c.V;
-        prep_.nx = c.NX; prep_.ny = c.NY; prep_.nz = c.NZ;
-        prep_.delta = c.p.seed_area_delta;
+template <typename T>
+void launch_densities(scft_ctx *c, int nact)
+{
+    if (c->nqslots > 0) {
+        hipLaunchKernelGGL(k_qsum<T>, dim3(c->nred, c->nqslots), dim3(kThreads), 0, c->stream,
+                           (const QSlot *)c->d_qslots);
+        hipLaunchKernelGGL(k_qfinal, dim3(nact), dim3(kThreads), 0, c->stream, (const FinArgs *)c->d_fin);
+    }
+    hipLaunchKernelGGL(k_density<T>, dim3(c->nred, nact), dim3(kThreads), 0, c->stream, (const DenArgs *)c->d_den);
+}
 
-        // ---- Q slots: dbk1/2 from q-dagger, dbk3/4 and the rest from q
-        // (SCFT.c:1271, 1314, 1357, 1399, 1422)
-        std::vector<const T *> ends;
-        FinArgs fin{};
-        for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
-            const int ns = d.ns_sp[sp];
-            if (ns <= 0) continue;
-            const T *src = (sp < 2) ? qd_[sp] : q_[sp];
-            ends.push_back(src + (size_t)(ns - 1) * V);
-            fin.qsp[fin.nq] = sp;
-            double coef = 0.0;
-            if (sp < 4) coef = c.p.vol_dbk[sp] / d.alpha[sp] * (1.0 - d.coating_vol);
-            else if (sp < 14) coef = c.p.vol_homo[sp - 4] / d.alpha[sp] * (1.0 - d.coating_vol);
-            else coef = c.p.mask ? c.p.vol_s[sp - 14] / d.alpha[sp] : 0.0;
-            fin.lnq_coef[fin.nq] = coef;
-            fin.nq++;
+void launch_update(scft_ctx *c, int nact)
+{
+    hipLaunchKernelGGL(k_update, dim3(c->nred, nact), dim3(kThreads), 0, c->stream, (const UpdArgs *)c->d_upd);
+    hipLaunchKernelGGL(k_final, dim3(nact), dim3(kThreads), 0, c->stream, (const FinArgs *)c->d_fin);
+    hipLaunchKernelGGL(k_pshift, dim3(grid_for(c->max_V, 512), nact), dim3(kThreads), 0, c->stream,
+                       (const FinArgs *)c->d_fin);
+}
+
+int maxV_active(scft_ctx *c)
+{
+    int m = 1;
+    for (int si : c->act) m = std::max(m, c->slots[si].V);
+    return m;
+}
+
+// phases: 1 propagate, 2 densities, 4 update, 8 copy scalars back
+int enqueue(scft_ctx *c, int phases)
+{
+    const int nact = int(c->act.size());
+    if (nact == 0) return SCFT_OK;
+    const bool f32 = c->tsize == 4;
+    if (phases & 1) {
+        if (f32) { launch_prep<float>(c, nact, maxV_active(c)); launch_steps<float>(c); }
+        else { launch_prep<double>(c, nact, maxV_active(c)); launch_steps<double>(c); }
+    }
+    if (phases & 2) {
+        if (f32) launch_densities<float>(c, nact);
+        else launch_densities<double>(c, nact);
+    }
+    if (phases & 4) launch_update(c, nact);
+    if (phases & 8)
+        HIPCHK(hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scalars) * c->nslots, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipGetLastError());
+    return SCFT_OK;
+}
+
+int run_iteration(scft_ctx *c, float mf, const double *noise_dev)
+{
+    if (c->dirty) RETCHK(build_schedule(c));
+    RETCHK(set_iteration_args(c, mf, noise_dev));
+    if (c->act.empty()) return SCFT_OK;
+    if (c->use_graph) {
+        if (!c->gexec) {
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue(c, 1 | 2 | 4 | 8);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc != SCFT_OK) return rc;
+            if (e != hipSuccess) {
+                set_err(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                return SCFT_EHIP;
+            }
+            c->graph = g;
+            HIPCHK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
         }
-        nq_ = fin.nq;
-        RETCHK(alloc(&ends_dev_, std::max<size_t>(1, ends.size())));
-        HIPCHK(hipMemcpy(ends_dev_, ends.data(), ends.size() * sizeof(T *), hipMemcpyHostToDevice));
-        fin.V = c.V;
-        fin.nblocks = c.nred;
-        fin.nact = d.n_active;
-        fin.qpart = c.qpart;
-        fin.dpart = c.dpart;
-        fin.upart = c.upart;
-        fin.out = c.scal;
-        HIPCHK(hipMemcpy(c.fin, &fin, sizeof(fin), hipMemcpyHostToDevice));
-
-        // ---- density terms (SCFT.c:1092-1513)
-        DenArgs<T> den;
-        std::memset(&den, 0, sizeof(den));
-        auto term = [&](int sp, int start, int stop, int field, double frac) {
-            DenTerm<T> &t = den.term[den.nterms++];
-            t.q = q_[sp];
-            t.qd = qd_[sp];
-            t.ns = d.ns_sp[sp];
-            t.start = start;
-            t.stop = stop;
-            t.field = field;
-            t.qslot = sp;
-            t.frac = frac;
-        };
-        const double cv = d.coating_vol;
-        for (int h = 0; h < 10; ++h)
-            if (d.ns_sp[4 + h] > 0) term(4 + h, 0, d.ns_sp[4 + h] - 1, h, c.p.vol_homo[h] * (1.0 - cv));
-        for (int b = 0; b < 4; ++b) {
-            if (d.ns_sp[b] <= 0) continue;
-            const double fr = c.p.vol_dbk[b] * (1.0 - cv);
-            term(b, 0, d.ns_x[b] - 1, kDbkField[b][0], fr);
-            term(b, d.ns_x[b] - 1, d.ns_sp[b] - 1, kDbkField[b][1], fr);
-        }
-        for (int s = 0; s < 3; ++s)
-            if (d.ns_sp[14 + s] > 0) term(14 + s, 0, d.ns_sp[14 + s] - 1, 10 + s, c.p.mask ? c.p.vol_s[s] : 0.0);
-        bool carries[13] = {false};
-        for (int i = 0; i < den.nterms; ++i) carries[den.term[i].field] = true;
-        for (int f = 0; f < 13; ++f)
-            if (carries[f]) den.phif[den.nphi++] = f;
-        for (int f = 0; f < 13; ++f)
-            for (int g = 0; g < 13; ++g) den.chi[f][g] = d.chi_eff[f][g];
-        den.V = c.V;
-        den.mask = c.p.mask;
-        den.Q = c.scal->Q;
-        den.w = c.w;
-        den.p = c.pf;
-        den.tot = c.tot;
-        den.phitot = c.phitot;
-        den.partial = c.dpart;
-        RETCHK(alloc(&den_dev_, 1));
-        HIPCHK(hipMemcpy(den_dev_, &den, sizeof(den), hipMemcpyHostToDevice));
-        return SCFT_OK;
+        HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    } else {
+        RETCHK(enqueue(c, 1 | 2 | 4 | 8));
     }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SCFT_OK;
+}
 
-    int propagate(unsigned long iter) override
-    {
-        scft_ctx &c = *c_;
-        PrepArgs pa = prep_;
-        pa.seed_mode = 0;
-        if (c.d.ns_sp[0] > 0 && c.p.enable_seed)
-            pa.seed_mode = (iter <= (unsigned long)c.p.seed_steps) ? 1 : 2;
-        hipLaunchKernelGGL(k_prep<T>, dim3(grid_for(c.V)), dim3(kThreads), 0, c.stream, pa, E_,
-                           c.d.ns_sp[0] > 0 ? q_[0] : nullptr);
-        for (const Launch &ln : launches_) {
-            hipLaunchKernelGGL(k_lineop<T>, dim3(ln.gx, ln.count), dim3(kThreads), ln.lds, c.stream,
-                               ops_dev_ + ln.first, ln.P);
-        }
-        HIPCHK(hipGetLastError());
-        return SCFT_OK;
+void fill_out(scft_ctx *c, int si, float mf, scft_iter_out *out)
+{
+    std::memset(out, 0, sizeof(*out));
+    const Scalars &s = c->scal_host[si];
+    const Slot &sl = c->slots[si];
+    out->free_energy = s.F;
+    out->rmsdev = s.rms;
+    out->dw_rms = s.dw_rms;
+    for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
+        if (sl.d.ns_sp[sp] <= 0) continue;
+        out->qsum[sp] = s.Q[sp];
+        const double qmod = std::pow(s.Q[sp], 2);   // SCFT.c:1862-1869
+        out->chempot[sp] = (qmod == 0) ? 0.0 : -std::log(qmod) / 2.0;
     }
-
-    int densities() override
-    {
-        scft_ctx &c = *c_;
-        if (nq_ > 0) {
-            hipLaunchKernelGGL(k_qsum<T>, dim3(c.nred, nq_), dim3(kThreads), 0, c.stream,
-                               (const T *const *)ends_dev_, c.V, c.qpart);
-            hipLaunchKernelGGL(k_qfinal, dim3(nq_), dim3(kThreads), 0, c.stream, (const FinArgs *)c.fin);
-        }
-        hipLaunchKernelGGL(k_density<T>, dim3(c.nred), dim3(kThreads), 0, c.stream,
-                           (const DenArgs<T> *)den_dev_);
-        HIPCHK(hipGetLastError());
-        return SCFT_OK;
-    }
-
-    int update(float mf, const double *noise_dev) override
-    {
-        scft_ctx &c = *c_;
-        UpdArgs u = c.upd_host;
-        u.amp = c.p.dt_noise * mf;
-        u.noise = (mf != 0.0f) ? noise_dev : nullptr;
-        HIPCHK(hipMemcpyAsync(c.upd, &u, sizeof(u), hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(k_update, dim3(c.nred), dim3(kThreads), 0, c.stream, (const UpdArgs *)c.upd);
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, c.stream, (const FinArgs *)c.fin);
-        hipLaunchKernelGGL(k_pshift, dim3(grid_for(c.V)), dim3(kThreads), 0, c.stream, c.pf, c.V,
-                           (const Scalars *)c.scal);
-        HIPCHK(hipGetLastError());
-        return SCFT_OK;
-    }
-
-    int set_mask(const double *mask_dev) override
-    {
-        scft_ctx &c = *c_;
-        for (int s = 0; s < 3; ++s)
-            if (c.d.ns_sp[14 + s] > 0)
-                hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(c.V)), dim3(kThreads), 0, c.stream,
-                                   mask_dev + (size_t)s * c.V, q_[14 + s], c.V);
-        HIPCHK(hipGetLastError());
-        return SCFT_OK;
-    }
-
-    int get_propagator(int sp, int which, int s, double *host) override
-    {
-        scft_ctx &c = *c_;
-        if (sp < 0 || sp >= SCFT_NSPECIES || c.d.ns_sp[sp] <= 0 || s < 0 || s >= c.d.ns_sp[sp]) {
-            set_err("scft_get_propagator: no such slice");
-            return SCFT_EINVAL;
-        }
-        const T *src = (which ? qd_[sp] : q_[sp]) + (size_t)s * c.V;
-        double *tmp = nullptr;
-        RETCHK(dev_alloc(&tmp, c.V));
-        hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(c.V)), dim3(kThreads), 0, c.stream, src, tmp, c.V);
-        hipError_t e = hipMemcpyAsync(host, tmp, sizeof(double) * c.V, hipMemcpyDeviceToHost, c.stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-        (void)hipFree(tmp);
-        if (e != hipSuccess) {
-            set_err(std::string("get_propagator: ") + hipGetErrorString(e));
-            return SCFT_EHIP;
-        }
-        return SCFT_OK;
-    }
-
-private:
-    struct Chain {
-        T *base;
-        T *tmp;
-        int nsteps, fa, fb, switch_at;
-    };
-    struct Launch {
-        int first, count, P, gx;
-        size_t lds;
-    };
-
-    template <typename X>
-    int alloc(X **p, size_t n)
-    {
-        RETCHK(dev_alloc(p, n));
-        allocs_.push_back(*p);
-        return SCFT_OK;
-    }
-
-    void fill(T *p, T v, size_t n)
-    {
-        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(kThreads), 0, c_->stream, p, v, n);
-    }
-
-    void add_chain(T *base, int nsteps, int fa, int fb, int switch_at)
-    {
-        chains_.push_back(Chain{base, nullptr, nsteps, fa, fb, switch_at});
-    }
-
-    static int lines_P(int L)
-    {
-        const int budget = 64 * 1024;
-        int P = budget / int(2 * L * sizeof(cpx<T>));
-        return std::max(1, std::min(P, 16));
-    }
-
-    int setup_axis(Axis &a, double alpha)
-    {
-        if (!plan_radices(a.L, a.rad)) {
-            set_err("transform length " + std::to_string(a.L) + " has too many factors");
-            return SCFT_EUNSUPPORTED;
-        }
-        if (size_t(2) * a.L * sizeof(cpx<T>) > 64 * 1024) {
-            set_err("transform length " + std::to_string(a.L) + " exceeds the LDS tile");
-            return SCFT_EUNSUPPORTED;
-        }
-        std::vector<double> k = axis_k(a.L, alpha);
-        std::vector<T> kt(k.begin(), k.end());
-        std::vector<cpx<T>> tw = twiddles<T>(a.L);
-        T *Kd = nullptr;
-        cpx<T> *twd = nullptr;
-        RETCHK(alloc(&Kd, a.L));
-        RETCHK(alloc(&twd, a.L));
-        HIPCHK(hipMemcpy(Kd, kt.data(), a.L * sizeof(T), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(twd, tw.data(), a.L * sizeof(cpx<T>), hipMemcpyHostToDevice));
-        a.K = Kd;
-        a.tw = twd;
-        return SCFT_OK;
-    }
-
-    scft_ctx *c_;
-    std::vector<void *> allocs_;
-    T *E_ = nullptr;
-    T *q_[SCFT_NSPECIES] = {};
-    T *qd_[SCFT_NSPECIES] = {};
-    std::vector<Chain> chains_;
-    std::vector<Axis> axes_;
-    std::vector<Launch> launches_;
-    int maxsteps_ = 0;
-    LineOp<T> *ops_dev_ = nullptr;
-    T **ends_dev_ = nullptr;
-    int nq_ = 0;
-    DenArgs<T> *den_dev_ = nullptr;
-    PrepArgs prep_{};
-};
+    out->mf = mf;
+}
 
 void free_ctx(scft_ctx *c)
 {
     if (!c) return;
-    c->eng.reset();
-    double *bufs[] = {c->w, c->pf, c->tot, c->phitot, c->mask, c->qpart, c->dpart, c->upart, c->noise};
-    for (double *b : bufs)
-        if (b) (void)hipFree(b);
-    if (c->scal) (void)hipFree(c->scal);
-    if (c->fin) (void)hipFree(c->fin);
-    if (c->upd) (void)hipFree(c->upd);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (void *p : c->allocs) (void)hipFree(p);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
-int fetch_out(scft_ctx *c, float mf, scft_iter_out *out)
+// Autotune the z operator for this context's size: time one step of every
+// chain of one slot with each candidate and keep the faster.
+int autotune_z(scft_ctx *c)
 {
-    HIPCHK(hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (!out) return SCFT_OK;
-    std::memset(out, 0, sizeof(*out));
-    const Scalars &s = *c->scal_host;
-    out->free_energy = s.F;
-    out->rmsdev = s.rms;
-    out->dw_rms = s.dw_rms;
-    for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
-        if (c->d.ns_sp[sp] <= 0) continue;
-        out->qsum[sp] = s.Q[sp];
-        const double qmod = std::pow(s.Q[sp], 2);
-        out->chempot[sp] = (qmod == 0) ? 0.0 : -std::log(qmod) / 2.0;
+    const scft_params &p = c->proto;
+    const double DS = 1.0 / (p.ns - 1);
+    const double DZ = p.lz / c->NZ;
+    const double az = DS / (DZ * DZ);
+    long long abits;
+    std::memcpy(&abits, &az, sizeof(abits));
+    auto key = std::make_tuple(c->NZ, p.mask ? 1 : 0, abits, int(c->tsize));
+    const int hs = c->NZ - c->NZ / 2;
+    const bool dense_ok = hs <= kDenseHMax;
+    // SCFT_ZKIND=fft|dense forces the z operator (tests use it to check both)
+    const char *env = std::getenv("SCFT_ZKIND");
+    if (env && std::strcmp(env, "fft") == 0) {
+        c->zkind = Z_FFT;
+        return SCFT_OK;
     }
-    out->mf = mf;
+    if (env && std::strcmp(env, "dense") == 0 && dense_ok) {
+        c->zkind = Z_DENSE;
+        return SCFT_OK;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) { c->zkind = it->second; return SCFT_OK; }
+    }
+    int choice = Z_FFT;
+    if (dense_ok) {
+        double best = 1e30;
+        for (int kind : {Z_FFT, Z_DENSE}) {
+            c->zkind = kind;
+            RETCHK(slot_init(c, c->slots[0], c->proto));
+            c->slots[0].active = true;
+            RETCHK(build_schedule(c));
+            // time the z launches of step 0 only
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            std::vector<Launch> zl;
+            for (auto &ln : c->launches) {
+                zl.push_back(ln);
+                break;
+            }
+            std::swap(zl, c->launches);
+            const bool f32 = c->tsize == 4;
+            for (int w = 0; w < 3; ++w) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
+            HIPCHK(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            std::swap(zl, c->launches);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (ms < best) { best = ms; choice = kind; }
+        }
+        c->slots[0].active = false;
+        c->slots[0].ready = false;
+        c->dirty = true;
+    }
+    c->zkind = choice;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = choice;
     return SCFT_OK;
 }
 
+Slot *get_slot(scft_ctx *c, int slot)
+{
+    if (!c || slot < 0 || slot >= c->nslots) {
+        set_err("bad slot index");
+        return nullptr;
+    }
+    return &c->slots[slot];
+}
+
 }  // namespace
+
+// ================================================================== C ABI
 
 extern "C" {
 
 const char *scft_last_error(void) { return g_err.c_str(); }
 
-int scft_create(const scft_params *p, int precision, scft_ctx **out)
+int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int precision, scft_ctx **out)
 {
-    if (!p || !out || (precision != SCFT_FP64 && precision != SCFT_FP32_SPLIT)) {
-        set_err("scft_create: bad argument");
+    if (!proto || !out || nslots < 1 || (precision != SCFT_FP64 && precision != SCFT_FP32_SPLIT)) {
+        set_err("scft_batch_create: bad argument");
         return SCFT_EINVAL;
     }
     *out = nullptr;
     scft_ctx *c = new scft_ctx();
-    c->p = *p;
     c->precision = precision;
-    int rc = scft_derive(p, &c->d);
+    c->tsize = precision == SCFT_FP64 ? 8 : 4;
+    c->proto = *proto;
+    int rc = scft_derive(proto, &c->dproto);
     if (rc != SCFT_OK) {
         set_err(rc == SCFT_EVOLUME ? "volumes do not sum to 1" : "invalid parameters");
         delete c;
         return rc;
     }
-    c->NX = c->d.nx;
-    c->NY = c->d.ny;
-    c->NZ = c->d.nz;
-    const size_t V = size_t(c->NX) * c->NY * c->NZ;
+    c->NY = c->dproto.ny;
+    c->NZ = c->dproto.nz;
+    c->max_nx = std::max(max_nx, c->dproto.nx);
+    const size_t V = size_t(c->max_nx) * c->NY * c->NZ;
     if (V > size_t(1) << 30) {
         set_err("grid too large");
         delete c;
         return SCFT_EUNSUPPORTED;
     }
-    c->V = int(V);
+    c->max_V = int(V);
+    c->nslots = nslots;
     c->nred = grid_for(V, kRedBlocks);
+    const char *g = std::getenv("SCFT_GRAPH");
+    c->use_graph = !(g && g[0] == '0');
     auto fail = [&](int r) {
         free_ctx(c);
         return r;
@@ -588,55 +1022,21 @@ int scft_create(const scft_params *p, int precision, scft_ctx **out)
         set_err("hipStreamCreate failed");
         return fail(SCFT_EHIP);
     }
-    if ((rc = dev_alloc(&c->w, 13 * V)) || (rc = dev_alloc(&c->pf, V)) || (rc = dev_alloc(&c->tot, 13 * V)) ||
-        (rc = dev_alloc(&c->phitot, V)) || (rc = dev_alloc(&c->mask, 3 * V)) ||
-        (rc = dev_alloc(&c->qpart, size_t(SCFT_NSPECIES) * c->nred)) || (rc = dev_alloc(&c->dpart, 2 * size_t(c->nred))) ||
-        (rc = dev_alloc(&c->upart, 2 * size_t(c->nred))) || (rc = dev_alloc(&c->scal, 1)) ||
-        (rc = dev_alloc(&c->fin, 1)) || (rc = dev_alloc(&c->upd, 1)))
-        return fail(rc);
-    if (hipHostMalloc(reinterpret_cast<void **>(&c->scal_host), sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
+    // the FFT tiles may use more than 64 KB of LDS on gfx950 (160 KB per workgroup)
+    const void *lk[] = {(const void *)k_lineop<double, double>, (const void *)k_lineop<float, double>,
+                        (const void *)k_lineop<double, float>, (const void *)k_lineop<float, float>};
+    for (const void *f : lk) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    c->slots.resize(nslots);
+    if ((rc = ctx_alloc(c, &c->scal, nslots))) return fail(rc);
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->scal_host), sizeof(Scalars) * nslots, hipHostMallocDefault) != hipSuccess) {
         set_err("hipHostMalloc failed");
         return fail(SCFT_ENOMEM);
     }
-    // zero state: fields, densities (absent chemistries stay 0, SCFT.c:785-843)
-    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(13 * V)), dim3(kThreads), 0, c->stream, c->w, 0.0, 13 * V);
-    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(V)), dim3(kThreads), 0, c->stream, c->pf, 0.0, V);
-    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(13 * V)), dim3(kThreads), 0, c->stream, c->tot, 0.0, 13 * V);
-    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(V)), dim3(kThreads), 0, c->stream, c->phitot, 0.0, V);
-    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(3 * V)), dim3(kThreads), 0, c->stream, c->mask, 0.0, 3 * V);
-    hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kThreads), 0, c->stream, reinterpret_cast<double *>(c->scal), 0.0,
-                       sizeof(Scalars) / sizeof(double));
-
-    // update arguments (constant per context)
-    UpdArgs &u = c->upd_host;
-    std::memset(&u, 0, sizeof(u));
-    u.V = c->V;
-    u.nx = c->NX;
-    u.ny = c->NY;
-    u.nz = c->NZ;
-    for (int f = 0; f < 13; ++f)
-        if (c->d.active[f]) u.act[u.nact++] = f;
-    for (int f = 0; f < 13; ++f) {
-        u.phibar[f] = c->d.phi_avg[f];
-        for (int g = 0; g < 13; ++g) u.chi[f][g] = c->d.chi_eff[f][g];
-    }
-    u.g1 = p->g1;
-    u.g2 = p->g1 / 6.0;
-    u.w = c->w;
-    u.p = c->pf;
-    u.tot = c->tot;
-    u.partial = c->upart;
-
-    if (precision == SCFT_FP64) {
-        auto e = std::make_unique<Engine<double>>(c);
-        rc = e->init();
-        c->eng = std::move(e);
-    } else {
-        auto e = std::make_unique<Engine<float>>(c);
-        rc = e->init();
-        c->eng = std::move(e);
-    }
-    if (rc != SCFT_OK) return fail(rc);
+    std::memset(c->scal_host, 0, sizeof(Scalars) * nslots);
+    fill_d(c, reinterpret_cast<double *>(c->scal), 0.0, sizeof(Scalars) / sizeof(double) * nslots);
+    for (auto &s : c->slots)
+        if ((rc = slot_alloc(c, s))) return fail(rc);
+    if ((rc = autotune_z(c))) return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         set_err("initialisation failed");
         return fail(SCFT_EHIP);
@@ -645,77 +1045,179 @@ int scft_create(const scft_params *p, int precision, scft_ctx **out)
     return SCFT_OK;
 }
 
+int scft_slot_init(scft_ctx *c, int slot, const scft_params *p)
+{
+    Slot *s = get_slot(c, slot);
+    if (!s || !p) return SCFT_EINVAL;
+    s->active = false;
+    RETCHK(slot_init(c, *s, *p));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SCFT_OK;
+}
+
+int scft_slot_activate(scft_ctx *c, int slot, int on)
+{
+    Slot *s = get_slot(c, slot);
+    if (!s) return SCFT_EINVAL;
+    if (on && !s->ready) {
+        set_err("slot not initialised");
+        return SCFT_EINVAL;
+    }
+    if (s->active != (on != 0)) {
+        s->active = on != 0;
+        c->dirty = true;
+    }
+    return SCFT_OK;
+}
+
+int scft_batch_size(const scft_ctx *c) { return c ? c->nslots : SCFT_EINVAL; }
+
+int scft_create(const scft_params *p, int precision, scft_ctx **out)
+{
+    RETCHK(scft_batch_create(p, 1, 0, precision, out));
+    int rc = scft_slot_init(*out, 0, p);
+    if (rc == SCFT_OK) rc = scft_slot_activate(*out, 0, 1);
+    if (rc != SCFT_OK) {
+        free_ctx(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
 int scft_destroy(scft_ctx *ctx)
 {
     if (!ctx) return SCFT_EINVAL;
-    (void)hipStreamSynchronize(ctx->stream);
     free_ctx(ctx);
     return SCFT_OK;
 }
 
-int scft_get_derived(const scft_ctx *ctx, scft_derived *out)
+int scft_slot_get_derived(const scft_ctx *c, int slot, scft_derived *out)
 {
-    if (!ctx || !out) return SCFT_EINVAL;
-    *out = ctx->d;
+    if (!c || !out || slot < 0 || slot >= c->nslots) return SCFT_EINVAL;
+    *out = c->slots[slot].ready ? c->slots[slot].d : c->dproto;
     return SCFT_OK;
 }
 
-int scft_set_fields(scft_ctx *c, const double *w, const double *p)
+int scft_get_derived(const scft_ctx *c, scft_derived *out) { return scft_slot_get_derived(c, 0, out); }
+
+int scft_slot_set_fields(scft_ctx *c, int slot, const double *w, const double *p)
 {
-    if (!c || !w || !p) return SCFT_EINVAL;
-    HIPCHK(hipMemcpyAsync(c->w, w, sizeof(double) * 13 * c->V, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->pf, p, sizeof(double) * c->V, hipMemcpyHostToDevice, c->stream));
+    Slot *s = get_slot(c, slot);
+    if (!s || !w || !p || !s->ready) return SCFT_EINVAL;
+    HIPCHK(hipMemcpyAsync(s->w, w, sizeof(double) * 13 * s->V, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(s->pf, p, sizeof(double) * s->V, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SCFT_OK;
 }
 
-int scft_get_fields(scft_ctx *c, double *w, double *p)
+int scft_slot_copy_fields(scft_ctx *c, int dst_slot, int src_slot)
 {
-    if (!c || !w || !p) return SCFT_EINVAL;
-    HIPCHK(hipMemcpyAsync(w, c->w, sizeof(double) * 13 * c->V, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(p, c->pf, sizeof(double) * c->V, hipMemcpyDeviceToHost, c->stream));
+    Slot *d = get_slot(c, dst_slot);
+    Slot *s = get_slot(c, src_slot);
+    if (!d || !s || !d->ready || !s->ready || d->V != s->V) {
+        set_err("slots differ in size or are not initialised");
+        return SCFT_EINVAL;
+    }
+    HIPCHK(hipMemcpyAsync(d->w, s->w, sizeof(double) * 13 * s->V, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d->pf, s->pf, sizeof(double) * s->V, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SCFT_OK;
 }
 
-int scft_set_mask(scft_ctx *c, const double *mask)
+int scft_slot_get_fields(scft_ctx *c, int slot, double *w, double *p)
 {
-    if (!c || !mask) return SCFT_EINVAL;
-    HIPCHK(hipMemcpyAsync(c->mask, mask, sizeof(double) * 3 * c->V, hipMemcpyHostToDevice, c->stream));
-    RETCHK(c->eng->set_mask(c->mask));
+    Slot *s = get_slot(c, slot);
+    if (!s || !w || !p || !s->ready) return SCFT_EINVAL;
+    HIPCHK(hipMemcpyAsync(w, s->w, sizeof(double) * 13 * s->V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(p, s->pf, sizeof(double) * s->V, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SCFT_OK;
 }
 
-int scft_get_densities(scft_ctx *c, double *phi, double *phi_tot)
+int scft_slot_set_mask(scft_ctx *c, int slot, const double *mask)
 {
-    if (!c || !phi) return SCFT_EINVAL;
-    HIPCHK(hipMemcpyAsync(phi, c->tot, sizeof(double) * 13 * c->V, hipMemcpyDeviceToHost, c->stream));
+    Slot *s = get_slot(c, slot);
+    if (!s || !mask || !s->ready) return SCFT_EINVAL;
+    HIPCHK(hipMemcpyAsync(s->mask, mask, sizeof(double) * 3 * s->V, hipMemcpyHostToDevice, c->stream));
+    // brush q starts from the mask every iteration (SCFT.c:2441); slices
+    // s >= 1 never overwrite slice 0, so setting it once is enough
+    for (int b = 0; b < 3; ++b) {
+        if (s->d.ns_sp[14 + b] <= 0) continue;
+        if (c->tsize == 8)
+            hipLaunchKernelGGL(k_convert<double>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream,
+                               s->mask + (size_t)b * s->V, (double *)s->q[14 + b], s->V);
+        else
+            hipLaunchKernelGGL(k_convert<float>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream,
+                               s->mask + (size_t)b * s->V, (float *)s->q[14 + b], s->V);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SCFT_OK;
+}
+
+int scft_slot_get_densities(scft_ctx *c, int slot, double *phi, double *phi_tot)
+{
+    Slot *s = get_slot(c, slot);
+    if (!s || !phi || !s->ready) return SCFT_EINVAL;
+    HIPCHK(hipMemcpyAsync(phi, s->tot, sizeof(double) * 13 * s->V, hipMemcpyDeviceToHost, c->stream));
     if (phi_tot)
-        HIPCHK(hipMemcpyAsync(phi_tot, c->phitot, sizeof(double) * c->V, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(phi_tot, s->phitot, sizeof(double) * s->V, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SCFT_OK;
 }
 
+int scft_slot_get_propagator(scft_ctx *c, int slot, int sp, int which, int idx, double *out)
+{
+    Slot *s = get_slot(c, slot);
+    if (!s || !out || !s->ready) return SCFT_EINVAL;
+    if (sp < 0 || sp >= SCFT_NSPECIES || s->d.ns_sp[sp] <= 0 || idx < 0 || idx >= s->d.ns_sp[sp]) {
+        set_err("scft_get_propagator: no such slice");
+        return SCFT_EINVAL;
+    }
+    const char *src = static_cast<const char *>(which ? s->qd[sp] : s->q[sp]) + (size_t)idx * s->V * c->tsize;
+    double *tmp = nullptr;
+    RETCHK(dev_alloc(&tmp, s->V));
+    if (c->tsize == 8)
+        hipLaunchKernelGGL(k_widen<double>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream, (const double *)src, tmp, s->V);
+    else
+        hipLaunchKernelGGL(k_widen<float>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream, (const float *)src, tmp, s->V);
+    hipError_t e = hipMemcpyAsync(out, tmp, sizeof(double) * s->V, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) {
+        set_err(std::string("get_propagator: ") + hipGetErrorString(e));
+        return SCFT_EHIP;
+    }
+    return SCFT_OK;
+}
+
+int scft_set_fields(scft_ctx *c, const double *w, const double *p) { return scft_slot_set_fields(c, 0, w, p); }
+int scft_get_fields(scft_ctx *c, double *w, double *p) { return scft_slot_get_fields(c, 0, w, p); }
+int scft_set_mask(scft_ctx *c, const double *m) { return scft_slot_set_mask(c, 0, m); }
+int scft_get_densities(scft_ctx *c, double *phi, double *t) { return scft_slot_get_densities(c, 0, phi, t); }
 int scft_get_propagator(scft_ctx *c, int sp, int which, int s, double *out)
 {
-    if (!c || !out) return SCFT_EINVAL;
-    return c->eng->get_propagator(sp, which, s, out);
+    return scft_slot_get_propagator(c, 0, sp, which, s, out);
 }
 
 int scft_propagate(scft_ctx *c, unsigned long iter)
 {
     if (!c) return SCFT_EINVAL;
-    return c->eng->propagate(iter);
+    if (c->dirty) RETCHK(build_schedule(c));
+    for (int si : c->act) c->slots[si].iter = iter;
+    RETCHK(set_iteration_args(c, 0.0f, nullptr));
+    return enqueue(c, 1);
 }
 
 int scft_densities(scft_ctx *c, double *qsum, double *chempot)
 {
     if (!c) return SCFT_EINVAL;
-    RETCHK(c->eng->densities());
-    if (qsum || chempot) {
+    if (c->dirty) RETCHK(build_schedule(c));
+    RETCHK(enqueue(c, 2 | 8));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->act.empty()) {
         scft_iter_out o;
-        RETCHK(fetch_out(c, 0.0f, &o));
+        fill_out(c, c->act[0], 0.0f, &o);
         for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
             if (qsum) qsum[sp] = o.qsum[sp];
             if (chempot) chempot[sp] = o.chempot[sp];
@@ -732,10 +1234,15 @@ static int upload_noise(scft_ctx *c, float mf, const double *noise, const double
         set_err("noise is required when mf != 0");
         return SCFT_EINVAL;
     }
-    const size_t n = size_t(c->d.n_active) * c->V;
-    if (!c->noise) RETCHK(dev_alloc(&c->noise, n));
-    HIPCHK(hipMemcpyAsync(c->noise, noise, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    *dev = c->noise;
+    if (c->act.size() != 1) {
+        set_err("noise is supported for a single active slot");
+        return SCFT_EINVAL;
+    }
+    Slot &s = c->slots[c->act[0]];
+    const size_t n = size_t(s.d.n_active) * s.V;
+    if (!s.noise) RETCHK(ctx_alloc(c, &s.noise, size_t(13) * c->max_V));
+    HIPCHK(hipMemcpyAsync(s.noise, noise, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    *dev = s.noise;
     return SCFT_OK;
 }
 
@@ -743,27 +1250,53 @@ int scft_update_fields(scft_ctx *c, unsigned long iter, float mf, const double *
 {
     (void)iter;
     if (!c) return SCFT_EINVAL;
+    if (c->dirty) RETCHK(build_schedule(c));
     const double *nd = nullptr;
     RETCHK(upload_noise(c, mf, noise, &nd));
-    RETCHK(c->eng->update(mf, nd));
-    return fetch_out(c, mf, out);
+    RETCHK(set_iteration_args(c, mf, nd));
+    RETCHK(enqueue(c, 4 | 8));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (out && !c->act.empty()) fill_out(c, c->act[0], mf, out);
+    return SCFT_OK;
 }
 
 int scft_iterate(scft_ctx *c, unsigned long iter, float mf, const double *noise, scft_iter_out *out)
 {
     if (!c) return SCFT_EINVAL;
+    if (c->dirty) RETCHK(build_schedule(c));
+    for (int si : c->act) c->slots[si].iter = iter;
     const double *nd = nullptr;
     RETCHK(upload_noise(c, mf, noise, &nd));
-    RETCHK(c->eng->propagate(iter));
-    RETCHK(c->eng->densities());
-    RETCHK(c->eng->update(mf, nd));
-    return fetch_out(c, mf, out);
+    RETCHK(run_iteration(c, mf, nd));
+    if (out && !c->act.empty()) fill_out(c, c->act[0], mf, out);
+    return SCFT_OK;
+}
+
+int scft_batch_iterate(scft_ctx *c, const unsigned long *iters, scft_iter_out *outs)
+{
+    if (!c) return SCFT_EINVAL;
+    if (c->dirty) RETCHK(build_schedule(c));
+    for (int si : c->act) c->slots[si].iter = iters ? iters[si] : 1ul << 40;
+    RETCHK(run_iteration(c, 0.0f, nullptr));
+    if (outs)
+        for (int si : c->act) fill_out(c, si, 0.0f, &outs[si]);
+    return SCFT_OK;
 }
 
 int scft_sync(scft_ctx *c)
 {
     if (!c) return SCFT_EINVAL;
     HIPCHK(hipStreamSynchronize(c->stream));
+    return SCFT_OK;
+}
+
+int scft_get_schedule(const scft_ctx *c, char *buf, int len)
+{
+    if (!c || !buf || len <= 0) return SCFT_EINVAL;
+    int nl = 0, nd = 0;
+    for (auto &ln : c->launches) (ln.kind ? nd : nl)++;
+    std::snprintf(buf, len, "z=%s launches/iter: lineop=%d dense=%d graph=%d", c->zkind == Z_DENSE ? "dense" : "fft", nl,
+                  nd, c->use_graph ? 1 : 0);
     return SCFT_OK;
 }
 

@@ -195,3 +195,47 @@ def test_fp32_split_close_to_fp64_oracle(abi, configs, orc, lib, name):
         print(name, "|dF| fp32-split vs fp64 oracle = %.3e" % dF)
         assert dF <= 1e-5
     ctx.close()
+
+
+def _golden():
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "c2_oracle.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("zkind", ["auto", "fft", "dense"])
+def test_c2_full_size_matches_golden(abi, configs, lib, zkind):
+    """C2 (256x1x128, MASK 1, NS 101) against the oracle's FP64 fixture
+    (tests/golden/make_golden.py): three iterations, 1e-10."""
+    import os
+    g = _golden()["c2_droplet_seed0"]
+    if zkind != "auto":
+        os.environ["SCFT_ZKIND"] = zkind
+    try:
+        p = configs.params_c2()
+        ctx = abi.Context(lib, p, abi.SCFT_FP64)
+        print(ctx.schedule())
+        w, pf = configs.droplet_fields(p, radius=6.0, seed=0)
+        ctx.set_fields(w, pf)
+        ctx.set_mask(configs.brush_mask(p))
+        for row in g["iters"]:
+            o = ctx.iterate(row["iter"])
+            assert _scalar_rel(o.free_energy, row["F"]) <= TOL, (row["iter"], o.free_energy, row["F"])
+            assert _scalar_rel(o.rmsdev, row["rmsdev"]) <= TOL
+            assert _scalar_rel(o.dw_rms, row["dw_rms"]) <= TOL
+            for sp in range(17):
+                assert _scalar_rel(o.chempot[sp], row["chempot"][sp]) <= TOL
+        wf, pfin = ctx.get_fields()
+        phi, _ = ctx.get_densities()
+        idx = g["sample_idx"]
+        for f, vals in g["w_samples"].items():
+            assert _rel(wf[int(f)][idx], vals) <= TOL
+            assert abs(np.sqrt((wf[int(f)] ** 2).sum()) - g["w_l2"][f]) <= TOL * g["w_l2"][f]
+        for f, vals in g["phi_samples"].items():
+            assert _rel(phi[int(f)][idx], vals) <= TOL
+        assert _rel(pfin[idx], g["p_samples"]) <= TOL
+        ctx.close()
+    finally:
+        os.environ.pop("SCFT_ZKIND", None)
