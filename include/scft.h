@@ -173,6 +173,18 @@ int scft_slot_get_propagator(scft_ctx *ctx, int slot, int sp, int which, int s, 
  * selects the dbk1 seed start; outs[slot] is filled for active slots.
  * No noise (MF = 0) in batch mode. */
 int scft_batch_iterate(scft_ctx *ctx, const unsigned long *iters, scft_iter_out *outs);
+/* Device bytes one slot of a batch with prototype p needs (fields,
+ * propagators, scratch); the serving layer admits solves by this footprint. */
+int scft_slot_bytes(const scft_params *p, int max_nx, int precision, size_t *bytes);
+/* Free and total memory of the current HIP device. */
+int scft_device_memory(size_t *free_bytes, size_t *total_bytes);
+/* Measurement hook: run every launch of one kernel kind of the current
+ * schedule (kind 0: z operator, 1: last axis) reps times between two HIP
+ * events on the context's stream; returns the mean launch duration (us)
+ * and the algorithmic bytes and flops of the mean launch.  Overwrites the
+ * chains' scratch and propagator slices, so call it between solves. */
+int scft_time_launches(scft_ctx *ctx, int kind, int reps, double *avg_us, double *bytes, double *flops,
+                       int *nlaunch);
 /* Human-readable schedule summary (z operator kind, launches per iteration). */
 int scft_get_schedule(const scft_ctx *ctx, char *buf, int len);
 

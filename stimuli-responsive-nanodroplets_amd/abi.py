@@ -165,6 +165,9 @@ def load_library(path=None):
         "scft_slot_get_propagator": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, dp], C.c_int),
         "scft_batch_iterate": ([vp, P(C.c_ulong), P(ScftIterOut)], C.c_int),
         "scft_get_schedule": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "scft_slot_bytes": ([P(ScftParams), C.c_int, C.c_int, P(C.c_size_t)], C.c_int),
+        "scft_device_memory": ([P(C.c_size_t), P(C.c_size_t)], C.c_int),
+        "scft_time_launches": ([vp, C.c_int, C.c_int, dp, dp, dp, P(C.c_int)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -345,3 +348,11 @@ class Batch:
         buf = C.create_string_buffer(256)
         check(self.lib, self.lib.scft_get_schedule(self.h, buf, 256), "scft_get_schedule")
         return buf.value.decode()
+
+    def time_launches(self, kind=0, reps=10):
+        """Mean duration (us), algorithmic bytes and flops of one launch of the
+        z-operator (kind 0) or last-axis (kind 1) kernel of this schedule."""
+        us, b, f, n = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        check(self.lib, self.lib.scft_time_launches(self.h, kind, reps, C.byref(us), C.byref(b), C.byref(f),
+                                                    C.byref(n)), "scft_time_launches")
+        return {"avg_us": us.value, "bytes": b.value, "flops": f.value, "launches": n.value}

@@ -219,6 +219,362 @@ __global__ void __launch_bounds__(kThreads) k_zdense(const DenseOp *__restrict__
     }
 }
 
+// ------------------------------------------------ dense z op on MFMA (FP64)
+//
+// v_mfma_f64_16x16x4f64 lane layout on gfx950 (checked on the device):
+//   A (16x4):  lane l holds A[l % 16][l / 16]
+//   B (4x16):  lane l holds B[l / 16][l % 16]
+//   D (16x16): lane l, register r holds D[l / 16 + 4 r][l % 16]
+// U = S Ms^T and V = A Ma^T for 16 rows per wave: the symmetric/antisymmetric
+// row parts are formed straight into A-fragment registers (E applied on
+// load), Ms^T / Ma^T are staged once per workgroup in LDS, HSP = hs padded
+// to a multiple of 16 (zero rows/columns).
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaWaves = 8;
+constexpr int kMfmaRows = 16 * kMfmaWaves;   // rows per workgroup
+
+// LDS row stride of the staged Ms^T / Ma^T: a half-wave reads fragment rows
+// t and NT + t; their offset NT*LDM doubles must be 16 (mod 32) doubles so
+// the two 16-double runs fall into disjoint bank halves.
+constexpr int mfma_ldm(int hsp)
+{
+    const int nt = hsp / 4;
+    for (int l = hsp; l < hsp + 64; ++l)
+        if ((nt * l) % 32 == 16) return l;
+    return hsp + 1;
+}
+
+template <typename Tin>
+__device__ __forceinline__ void load4(const Tin *p, double (&v)[4])
+{
+    if constexpr (sizeof(Tin) == 4) {
+        const float4 f = *reinterpret_cast<const float4 *>(p);
+        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+    } else {
+        const double2 a = *reinterpret_cast<const double2 *>(p);
+        const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+}
+
+// The K index j of the products is a summation index, so any bijection
+// between (t, l/16) and j serves if A and B use the same one: lane group
+// l/16 takes the contiguous quarter j = (l/16) NT + t, which turns each
+// lane's A-fragment loads into runs of contiguous vector loads.
+template <typename Tin, typename Tout, int HSP>
+__global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *__restrict__ ops)
+{
+    constexpr int NT = HSP / 4;    // K steps
+    constexpr int NK = HSP / 16;   // output column tiles
+    constexpr int LDM = mfma_ldm(HSP);
+    const DenseOp &op = ops[blockIdx.y];
+    const int row0 = blockIdx.x * kMfmaRows;
+    if (row0 >= op.nrows) return;
+    __shared__ double Ms[HSP * LDM];
+    __shared__ double Ma[HSP * LDM];
+    const int n = op.n, hs = op.hs, ha = op.ha;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < HSP * HSP; e += 64 * kMfmaWaves) {
+        const int j = e / HSP, k = e - j * HSP;
+        Ms[j * LDM + k] = (j < hs && k < hs) ? op.MsT[(size_t)j * op.ldm + k] : 0.0;
+        Ma[j * LDM + k] = (j < ha && k < ha) ? op.MaT[(size_t)j * op.ldm + k] : 0.0;
+    }
+    const Tin *__restrict__ src = static_cast<const Tin *>(op.src);
+    const double *__restrict__ ep = op.epre;
+    Tout *__restrict__ dst = static_cast<Tout *>(op.dst);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lj = lane >> 4;
+    const int rbase = row0 + wave * 16;
+    double sf[NT], af[NT];
+    {
+        const int row = rbase + li;
+        const bool ok = row < op.nrows;
+        const size_t base = (size_t)(ok ? row : 0) * n;
+        const int j0 = lj * NT;
+        if (ok && hs == HSP && ha == HSP && (n % 4) == 0) {
+            // full halves, n = 2 HSP: x1 runs up from j0, x2 runs down from n-1-j0
+#pragma unroll
+            for (int t = 0; t < NT; t += 4) {
+                double x1[4], x2[4];
+                load4<Tin>(src + base + j0 + t, x1);
+                load4<Tin>(src + base + n - 4 - j0 - t, x2);   // x2[3-i] = x[n-1-(j0+t+i)]
+                if (ep) {
+                    double e1[4], e2[4];
+                    load4<double>(ep + base + j0 + t, e1);
+                    load4<double>(ep + base + n - 4 - j0 - t, e2);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) { x1[i] *= e1[i]; x2[i] *= e2[i]; }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    sf[t + i] = x1[i] + x2[3 - i];
+                    af[t + i] = x1[i] - x2[3 - i];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = j0 + t;
+                double s = 0.0, a = 0.0;
+                if (ok && j < hs) {
+                    double x1 = double(src[base + j]);
+                    if (ep) x1 *= ep[base + j];
+                    if (j < ha) {
+                        double x2 = double(src[base + n - 1 - j]);
+                        if (ep) x2 *= ep[base + n - 1 - j];
+                        s = x1 + x2;
+                        a = x1 - x2;
+                    } else {
+                        s = x1;   // middle point of an odd line
+                    }
+                }
+                sf[t] = s;
+                af[t] = a;
+            }
+        }
+    }
+    __syncthreads();
+    d4_t U[NK], W[NK];
+#pragma unroll
+    for (int kt = 0; kt < NK; ++kt) {
+        U[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
+        W[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const double *ms = Ms + (lj * NT + t) * LDM + li;
+        const double *ma = Ma + (lj * NT + t) * LDM + li;
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) {
+            U[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[t], ms[16 * kt], U[kt], 0, 0, 0);
+            W[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], ma[16 * kt], W[kt], 0, 0, 0);
+        }
+    }
+    // D layout: register r of lane l is row l/16 + 4r, column l%16 of the tile
+#pragma unroll
+    for (int kt = 0; kt < NK; ++kt) {
+        const int k = 16 * kt + li;
+        if (k >= hs) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rbase + lj + 4 * r;
+            if (row >= op.nrows) continue;
+            const size_t base = (size_t)row * n;
+            const double u = U[kt][r], v = W[kt][r];
+            double y1 = u + v;
+            if (op.epost) y1 *= op.epost[base + k];
+            dst[base + k] = Tout(y1);
+            if (k < ha) {
+                double y2 = u - v;
+                if (op.epost) y2 *= op.epost[base + n - 1 - k];
+                dst[base + n - 1 - k] = Tout(y2);
+            }
+        }
+    }
+}
+
+// ------------------------------------------- periodic FFT, one line per wave
+//
+// Periodic axes (x, y, and z with MASK 0) of lengths known at compile time.
+// A workgroup of 8 wavefronts takes 16 consecutive real grid lines (16
+// contiguous doubles per grid row for strided x lines), packs them two per
+// complex line and gives each wavefront one complex line.  Stockham passes
+// have compile-time radices, so all index arithmetic folds to shifts and
+// multiplies; K is applied in the last forward pass, 1/L and E on store.
+
+template <int L> struct FftPlan;
+template <> struct FftPlan<16> { static constexpr int n = 2; static constexpr int R[2] = {4, 4}; };
+template <> struct FftPlan<32> { static constexpr int n = 3; static constexpr int R[3] = {4, 4, 2}; };
+template <> struct FftPlan<48> { static constexpr int n = 3; static constexpr int R[3] = {4, 4, 3}; };
+template <> struct FftPlan<64> { static constexpr int n = 3; static constexpr int R[3] = {4, 4, 4}; };
+template <> struct FftPlan<80> { static constexpr int n = 3; static constexpr int R[3] = {4, 4, 5}; };
+template <> struct FftPlan<96> { static constexpr int n = 4; static constexpr int R[4] = {4, 4, 2, 3}; };
+template <> struct FftPlan<128> { static constexpr int n = 4; static constexpr int R[4] = {4, 4, 4, 2}; };
+template <> struct FftPlan<160> { static constexpr int n = 4; static constexpr int R[4] = {4, 4, 2, 5}; };
+template <> struct FftPlan<192> { static constexpr int n = 4; static constexpr int R[4] = {4, 4, 4, 3}; };
+template <> struct FftPlan<256> { static constexpr int n = 4; static constexpr int R[4] = {4, 4, 4, 4}; };
+template <> struct FftPlan<320> { static constexpr int n = 4; static constexpr int R[4] = {4, 4, 4, 5}; };
+template <> struct FftPlan<384> { static constexpr int n = 5; static constexpr int R[5] = {4, 4, 4, 2, 3}; };
+template <> struct FftPlan<512> { static constexpr int n = 5; static constexpr int R[5] = {4, 4, 4, 4, 2}; };
+
+constexpr int kFftWaves = 8;
+
+template <int L, int P>
+struct PassInfo {
+    static constexpr int R = FftPlan<L>::R[P];
+    static constexpr int ns()
+    {
+        int v = 1;
+        for (int i = 0; i < P; ++i) v *= FftPlan<L>::R[i];
+        return v;
+    }
+    static constexpr int Ns = ns();
+    static constexpr int nb = L / R;
+    static constexpr int per = (nb + 63) / 64;   // butterflies per lane
+};
+
+template <int R>
+__device__ __forceinline__ void dft_small(cpx<double> (&v)[R], double sg)
+{
+    if constexpr (R == 2) {
+        cpx<double> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        cpx<double> a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
+        cpx<double> t = {v[0].x - 0.5 * a.x, v[0].y - 0.5 * a.y};
+        cpx<double> u = cmul_si(b, sg * Consts<double>::s3);
+        v[0] = cadd(v[0], a);
+        v[1] = cadd(t, u);
+        v[2] = csub(t, u);
+    } else if constexpr (R == 4) {
+        cpx<double> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        cpx<double> s13 = cadd(v[1], v[3]), d13 = cmul_si(csub(v[1], v[3]), sg);
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, d13);
+    } else {   // R == 5
+        const double c1 = Consts<double>::c51, c2 = Consts<double>::c52;
+        const double s1 = Consts<double>::s51, s2 = Consts<double>::s52;
+        cpx<double> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        cpx<double> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        cpx<double> t1 = {v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y};
+        cpx<double> t2 = {v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y};
+        cpx<double> u1 = cmul_si(cpx<double>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y}, sg);
+        cpx<double> u2 = cmul_si(cpx<double>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y}, sg);
+        v[0] = cpx<double>{v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y};
+        v[1] = cadd(t1, u1);
+        v[4] = csub(t1, u1);
+        v[2] = cadd(t2, u2);
+        v[3] = csub(t2, u2);
+    }
+}
+
+// The wave owns its line: LDS operations of one wavefront are performed in
+// order, so a wavefront-scope fence (a compiler barrier) orders the passes.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Stockham pass of the wave's line `buf` (in place: read all, sync, write all).
+template <int L, int P, int SIGN, bool APPLY_K>
+__device__ __forceinline__ void wave_pass(cpx<double> *buf, const cpx<double> *tw, const double *K, int lane)
+{
+    using PI = PassInfo<L, P>;
+    constexpr int R = PI::R, Ns = PI::Ns, nb = PI::nb, per = PI::per;
+    constexpr int tstep = L / (Ns * R);
+    cpx<double> v[per][R];
+#pragma unroll
+    for (int b = 0; b < per; ++b) {
+        const int j = lane + 64 * b;
+        if (j < nb) {
+            const int k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cpx<double> x = buf[j + r * nb];
+                if (r > 0) {
+                    cpx<double> t = tw[r * k * tstep];
+                    if (SIGN > 0) t.y = -t.y;
+                    x = cmul(x, t);
+                }
+                v[b][r] = x;
+            }
+            dft_small<R>(v[b], double(SIGN));
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < per; ++b) {
+        const int j = lane + 64 * b;
+        if (j < nb) {
+            const int k = j % Ns;
+            const int od = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cpx<double> x = v[b][r];
+                if (APPLY_K) {
+                    const double kk = K[od + r * Ns];
+                    x.x *= kk;
+                    x.y *= kk;
+                }
+                buf[od + r * Ns] = x;
+            }
+        }
+    }
+    wave_sync();
+}
+
+template <int L, int P, int SIGN>
+__device__ __forceinline__ void wave_fft(cpx<double> *buf, const cpx<double> *tw, const double *K, int lane)
+{
+    if constexpr (P < FftPlan<L>::n) {
+        constexpr bool last_fwd = (SIGN < 0) && (P == FftPlan<L>::n - 1);
+        wave_pass<L, P, SIGN, last_fwd>(buf, tw, K, lane);
+        wave_fft<L, P + 1, SIGN>(buf, tw, K, lane);
+    }
+}
+
+template <int L, typename Tin, typename Tout>
+__global__ void __launch_bounds__(64 * kFftWaves) k_fftwave(const LineOp *__restrict__ ops)
+{
+    constexpr int NL = 2 * kFftWaves;   // real lines per workgroup
+    const LineOp &op = ops[blockIdx.y];
+    const int g0 = blockIdx.x * NL;
+    if (g0 >= op.nlines) return;
+    // line stride L + 1: the 16 lanes that touch one grid row hit 16 different bank groups
+    constexpr int LS = L + 1;
+    __shared__ cpx<double> buf[kFftWaves * LS];
+    __shared__ cpx<double> tw[L];
+    __shared__ double K[L];
+    __shared__ int lbase[NL];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nl = min(NL, op.nlines - g0);
+    const Tin *__restrict__ src = static_cast<const Tin *>(op.src);
+    Tout *__restrict__ dst = static_cast<Tout *>(op.dst);
+    for (int e = tid; e < L; e += 64 * kFftWaves) {
+        tw[e] = op.tw[e];
+        K[e] = op.K[e];
+    }
+    if (tid < NL) lbase[tid] = tid < nl ? line_base(op, g0 + tid) : 0;
+    __syncthreads();
+    const bool contiguous = (op.stride == 1);
+    const int stride = op.stride;
+    for (int e = tid; e < NL * L; e += 64 * kFftWaves) {
+        int r, t;
+        if (contiguous) { r = e / L; t = e - r * L; }
+        else { t = e / NL; r = e - t * NL; }
+        double v = 0.0;
+        if (r < nl) {
+            const int a = lbase[r] + t * stride;
+            v = double(src[a]);
+            if (op.epre) v *= op.epre[a];
+        }
+        reinterpret_cast<double *>(&buf[(r >> 1) * LS + t])[r & 1] = v;
+    }
+    __syncthreads();
+    cpx<double> *line = buf + wave * LS;
+    wave_fft<L, 0, -1>(line, tw, K, lane);
+    wave_fft<L, 0, +1>(line, tw, K, lane);
+    __syncthreads();   // the store mixes lines of different waves
+    constexpr double scale = 1.0 / L;
+    for (int e = tid; e < NL * L; e += 64 * kFftWaves) {
+        int r, t;
+        if (contiguous) { r = e / L; t = e - r * L; }
+        else { t = e / NL; r = e - t * NL; }
+        if (r >= nl) continue;
+        const int a = lbase[r] + t * stride;
+        double v = reinterpret_cast<const double *>(&buf[(r >> 1) * LS + t])[r & 1] * scale;
+        if (op.epost) v *= op.epost[a];
+        dst[a] = Tout(v);
+    }
+}
+
 // ------------------------------------------------------------------ prep
 
 struct PrepArgs {

@@ -192,12 +192,25 @@ struct Slot {
     std::vector<QSlot> qslots;
 };
 
+enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3 };
+
 struct Launch {
-    int kind;        // 0 lineop, 1 dense
+    int kind;        // LaunchKind
     int pos;         // 0 first, 1 middle, 2 last, 3 only
     int first, count, P, gx;
+    int L = 0;       // transform length (FFT kinds)
+    int hsp = 0;     // padded dense half width (MFMA kind)
     size_t lds;
 };
+
+bool fftwave_supported(int L)
+{
+    switch (L) {
+    case 16: case 32: case 48: case 64: case 80: case 96: case 128: case 160:
+    case 192: case 256: case 320: case 384: case 512: return true;
+    default: return false;
+    }
+}
 
 struct scft_ctx {
     int precision = SCFT_FP64;
@@ -212,6 +225,8 @@ struct scft_ctx {
     std::vector<std::unique_ptr<FftTable>> ffts;
     std::vector<std::unique_ptr<DenseTable>> denses;
     int zkind = Z_FFT;
+    bool fft_wave = true;            // wave-per-line FFT for supported periodic lengths
+    bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
     Scalars *scal = nullptr;         // [nslots] device
     Scalars *scal_host = nullptr;    // [nslots] pinned
     // schedule of the active set
@@ -626,11 +641,8 @@ int build_schedule(scft_ctx *c)
             const int axis = axes[ai];
             const int pos = (nax == 1) ? 3 : (ai == 0 ? 0 : (ai == nax - 1 ? 2 : 1));
             const bool dense = (axis == 2 && c->zkind == Z_DENSE);
-            Launch ln{};
-            ln.kind = dense ? 1 : 0;
-            ln.pos = pos;
-            ln.first = dense ? int(dops.size()) : int(lops.size());
-            int maxl = 0, maxL = 1;
+            std::vector<LineOp> tl;     // this step/axis, every live chain
+            std::vector<DenseOp> td;
             for (int si : c->act) {
                 Slot &s = c->slots[si];
                 const size_t V = s.V;
@@ -653,8 +665,7 @@ int build_schedule(scft_ctx *c)
                         op.ha = s.dz->ha;
                         op.ldm = s.dz->ldm;
                         op.nrows = s.NX * s.NY;
-                        dops.push_back(op);
-                        maxl = std::max(maxl, op.nrows);
+                        td.push_back(op);
                         continue;
                     }
                     const FftTable *ft = axis == 2 ? s.fz : (axis == 1 ? s.fy : s.fx);
@@ -682,21 +693,57 @@ int build_schedule(scft_ctx *c)
                         op.nlines = s.NY * s.NZ;
                         op.c1 = s.NY * s.NZ; op.s0 = 1; op.s1 = 0; op.stride = s.NY * s.NZ;
                     }
-                    lops.push_back(op);
-                    maxl = std::max(maxl, op.nlines);
-                    maxL = std::max(maxL, op.L);
+                    tl.push_back(op);
                 }
             }
-            ln.count = (dense ? int(dops.size()) : int(lops.size())) - ln.first;
-            if (ln.count == 0) continue;
-            if (dense) {
-                ln.gx = (maxl + kDenseTM - 1) / kDenseTM;
-            } else {
-                ln.P = lines_P(maxL);
-                ln.lds = size_t(2) * ln.P * maxL * sizeof(cpx<double>);
-                ln.gx = (maxl + 2 * ln.P - 1) / (2 * ln.P);
+            if (dense && !td.empty()) {
+                Launch ln{};
+                ln.pos = pos;
+                ln.first = int(dops.size());
+                ln.count = int(td.size());
+                int maxl = 0;
+                for (auto &op : td) maxl = std::max(maxl, op.nrows);
+                const int hsp = ((td[0].hs + 15) / 16) * 16;
+                if (c->dense_mfma && hsp <= 64) {
+                    ln.kind = LK_DENSE_MFMA;
+                    ln.hsp = hsp;
+                    ln.gx = (maxl + kMfmaRows - 1) / kMfmaRows;
+                } else {
+                    ln.kind = LK_DENSE;
+                    ln.gx = (maxl + kDenseTM - 1) / kDenseTM;
+                }
+                dops.insert(dops.end(), td.begin(), td.end());
+                c->launches.push_back(ln);
             }
-            c->launches.push_back(ln);
+            // one launch per transform length (slots of a batch may differ in NX)
+            std::vector<int> Ls;
+            for (auto &op : tl)
+                if (std::find(Ls.begin(), Ls.end(), op.L) == Ls.end()) Ls.push_back(op.L);
+            for (int L : Ls) {
+                Launch ln{};
+                ln.pos = pos;
+                ln.first = int(lops.size());
+                int maxl = 0;
+                bool dirichlet = false;
+                for (auto &op : tl)
+                    if (op.L == L) {
+                        lops.push_back(op);
+                        maxl = std::max(maxl, op.nlines);
+                        dirichlet |= op.dirichlet != 0;
+                    }
+                ln.count = int(lops.size()) - ln.first;
+                ln.L = L;
+                if (c->fft_wave && !dirichlet && fftwave_supported(L)) {
+                    ln.kind = LK_FFTWAVE;
+                    ln.gx = (maxl + 2 * kFftWaves - 1) / (2 * kFftWaves);
+                } else {
+                    ln.kind = LK_LINEOP;
+                    ln.P = lines_P(L);
+                    ln.lds = size_t(2) * ln.P * L * sizeof(cpx<double>);
+                    ln.gx = (maxl + 2 * ln.P - 1) / (2 * ln.P);
+                }
+                c->launches.push_back(ln);
+            }
         }
     }
     RETCHK(upload(c, &c->lineops, &c->lineops_cap, lops));
@@ -769,28 +816,61 @@ void launch_prep(scft_ctx *c, int nact, int maxV)
                        (const PrepArgs *)c->d_prep);
 }
 
+// Typed launchers: the axis position fixes the storage types (first axis
+// reads the T slice, the last writes the T slice, the rest use FP64 tmp).
+template <typename Tin, typename Tout>
+void launch_fftwave(scft_ctx *c, const Launch &ln)
+{
+    const LineOp *ops = c->lineops + ln.first;
+    dim3 g(ln.gx, ln.count), b(64 * kFftWaves);
+    hipStream_t st = c->stream;
+    switch (ln.L) {
+#define SCFT_FFTW_CASE(LL) \
+    case LL: hipLaunchKernelGGL((k_fftwave<LL, Tin, Tout>), g, b, 0, st, ops); break;
+    SCFT_FFTW_CASE(16) SCFT_FFTW_CASE(32) SCFT_FFTW_CASE(48) SCFT_FFTW_CASE(64) SCFT_FFTW_CASE(80)
+    SCFT_FFTW_CASE(96) SCFT_FFTW_CASE(128) SCFT_FFTW_CASE(160) SCFT_FFTW_CASE(192) SCFT_FFTW_CASE(256)
+    SCFT_FFTW_CASE(320) SCFT_FFTW_CASE(384) SCFT_FFTW_CASE(512)
+#undef SCFT_FFTW_CASE
+    default: break;
+    }
+}
+
+template <typename Tin, typename Tout>
+void launch_one(scft_ctx *c, const Launch &ln)
+{
+    dim3 g(ln.gx, ln.count);
+    hipStream_t st = c->stream;
+    switch (ln.kind) {
+    case LK_DENSE_MFMA: {
+        const DenseOp *ops = c->denseops + ln.first;
+        const dim3 b(64 * kMfmaWaves);
+        if (ln.hsp == 16) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 16>), g, b, 0, st, ops);
+        else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 32>), g, b, 0, st, ops);
+        else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 48>), g, b, 0, st, ops);
+        else hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 64>), g, b, 0, st, ops);
+        break;
+    }
+    case LK_DENSE:
+        hipLaunchKernelGGL((k_zdense<Tin, Tout>), g, dim3(kThreads), 0, st, c->denseops + ln.first);
+        break;
+    case LK_FFTWAVE:
+        launch_fftwave<Tin, Tout>(c, ln);
+        break;
+    default:
+        hipLaunchKernelGGL((k_lineop<Tin, Tout>), g, dim3(kThreads), ln.lds, st, c->lineops + ln.first, ln.P);
+        break;
+    }
+}
+
 template <typename T>
 void launch_steps(scft_ctx *c)
 {
     for (const Launch &ln : c->launches) {
-        if (ln.kind == 1) {
-            const DenseOp *ops = c->denseops + ln.first;
-            dim3 g(ln.gx, ln.count);
-            switch (ln.pos) {
-            case 0: hipLaunchKernelGGL((k_zdense<T, double>), g, dim3(kThreads), 0, c->stream, ops); break;
-            case 1: hipLaunchKernelGGL((k_zdense<double, double>), g, dim3(kThreads), 0, c->stream, ops); break;
-            case 2: hipLaunchKernelGGL((k_zdense<double, T>), g, dim3(kThreads), 0, c->stream, ops); break;
-            default: hipLaunchKernelGGL((k_zdense<T, T>), g, dim3(kThreads), 0, c->stream, ops); break;
-            }
-        } else {
-            const LineOp *ops = c->lineops + ln.first;
-            dim3 g(ln.gx, ln.count);
-            switch (ln.pos) {
-            case 0: hipLaunchKernelGGL((k_lineop<T, double>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
-            case 1: hipLaunchKernelGGL((k_lineop<double, double>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
-            case 2: hipLaunchKernelGGL((k_lineop<double, T>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
-            default: hipLaunchKernelGGL((k_lineop<T, T>), g, dim3(kThreads), ln.lds, c->stream, ops, ln.P); break;
-            }
+        switch (ln.pos) {
+        case 0: launch_one<T, double>(c, ln); break;
+        case 1: launch_one<double, double>(c, ln); break;
+        case 2: launch_one<double, T>(c, ln); break;
+        default: launch_one<T, T>(c, ln); break;
         }
     }
 }
@@ -1014,6 +1094,12 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     c->nred = grid_for(V, kRedBlocks);
     const char *g = std::getenv("SCFT_GRAPH");
     c->use_graph = !(g && g[0] == '0');
+    // kernel variants (tests force each one): SCFT_XFFT=lds selects the
+    // generic LDS FFT for periodic axes, SCFT_DENSE=valu the VALU dense z op
+    const char *xf = std::getenv("SCFT_XFFT");
+    c->fft_wave = !(xf && std::strcmp(xf, "lds") == 0);
+    const char *dm = std::getenv("SCFT_DENSE");
+    c->dense_mfma = !(dm && std::strcmp(dm, "valu") == 0);
     auto fail = [&](int r) {
         free_ctx(c);
         return r;
@@ -1071,6 +1157,34 @@ int scft_slot_activate(scft_ctx *c, int slot, int on)
 }
 
 int scft_batch_size(const scft_ctx *c) { return c ? c->nslots : SCFT_EINVAL; }
+
+int scft_slot_bytes(const scft_params *p, int max_nx, int precision, size_t *bytes)
+{
+    if (!p || !bytes) return SCFT_EINVAL;
+    scft_derived d;
+    RETCHK(scft_derive(p, &d));
+    const size_t V = size_t(std::max(max_nx, d.nx)) * d.ny * d.nz;
+    const size_t ts = precision == SCFT_FP64 ? 8 : 4;
+    size_t b = 44 * V * sizeof(double);                 // w, p, tot, phitot, mask, E
+    size_t nchains = 0;
+    for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
+        if (d.ns_sp[sp] <= 0) continue;
+        const bool homo = sp >= 4 && sp < 14;
+        b += (homo ? 1 : 2) * size_t(d.ns_sp[sp]) * V * ts;   // q (and q-dagger)
+        nchains += homo ? 1 : 2;
+    }
+    b += nchains * V * sizeof(double);                  // per-chain axis scratch
+    b += (size_t(SCFT_NSPECIES) + 4) * kRedBlocks * sizeof(double);
+    *bytes = b;
+    return SCFT_OK;
+}
+
+int scft_device_memory(size_t *free_bytes, size_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return SCFT_EINVAL;
+    HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+    return SCFT_OK;
+}
 
 int scft_create(const scft_params *p, int precision, scft_ctx **out)
 {
@@ -1290,13 +1404,81 @@ int scft_sync(scft_ctx *c)
     return SCFT_OK;
 }
 
+// Time every launch of one kernel kind of the current schedule (one
+// iteration's worth), `reps` times, between two events on the context's
+// stream; report the mean launch duration and the algorithmic bytes and
+// flops of the mean launch.  kind: 0 = z operator, 1 = last (x) axis.
+int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *bytes, double *flops,
+                       int *nlaunch)
+{
+    if (!c || reps < 1 || !avg_us || !bytes || !flops || !nlaunch) return SCFT_EINVAL;
+    if (c->dirty) RETCHK(build_schedule(c));
+    std::vector<Launch> sel;
+    double B = 0.0, Fl = 0.0;
+    const double ts = double(c->tsize);
+    for (const Launch &ln : c->launches) {
+        const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA) ||
+                          ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE) && ln.pos == 0);
+        if ((kind == 0) != is_z) continue;
+        if (kind == 1 && ln.pos != 2 && ln.pos != 3) continue;
+        sel.push_back(ln);
+        // algorithmic traffic: read src + E, write dst (T at the slice end, FP64 scratch inside)
+        const double in_b = (ln.pos == 0 || ln.pos == 3) ? ts : 8.0;
+        const double out_b = (ln.pos == 2 || ln.pos == 3) ? ts : 8.0;
+        if (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA) {
+            std::vector<DenseOp> ops(ln.count);
+            HIPCHK(hipMemcpy(ops.data(), c->denseops + ln.first, ln.count * sizeof(DenseOp), hipMemcpyDeviceToHost));
+            for (auto &op : ops) {
+                const double elems = double(op.nrows) * op.n;
+                B += elems * (in_b + out_b + 8.0);
+                Fl += 2.0 * op.nrows * (double(op.hs) * op.hs + double(op.ha) * op.ha);
+            }
+        } else {
+            std::vector<LineOp> ops(ln.count);
+            HIPCHK(hipMemcpy(ops.data(), c->lineops + ln.first, ln.count * sizeof(LineOp), hipMemcpyDeviceToHost));
+            for (auto &op : ops) {
+                const double elems = double(op.nlines) * op.n;
+                B += elems * (in_b + out_b + 8.0);
+                // radix-2-equivalent count of a complex transform pair: 2 x 5 L log2 L per packed line pair
+                Fl += (op.nlines / 2.0) * 2.0 * 5.0 * op.L * std::log2(double(op.L));
+            }
+        }
+    }
+    *nlaunch = int(sel.size());
+    if (sel.empty()) {
+        *avg_us = *bytes = *flops = 0.0;
+        return SCFT_OK;
+    }
+    std::swap(sel, c->launches);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const bool f32 = c->tsize == 4;
+    f32 ? launch_steps<float>(c) : launch_steps<double>(c);   // warm
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    std::swap(sel, c->launches);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double nl = double(*nlaunch);
+    *avg_us = 1000.0 * ms / (reps * nl);
+    *bytes = B / nl;
+    *flops = Fl / nl;
+    return SCFT_OK;
+}
+
 int scft_get_schedule(const scft_ctx *c, char *buf, int len)
 {
     if (!c || !buf || len <= 0) return SCFT_EINVAL;
-    int nl = 0, nd = 0;
-    for (auto &ln : c->launches) (ln.kind ? nd : nl)++;
-    std::snprintf(buf, len, "z=%s launches/iter: lineop=%d dense=%d graph=%d", c->zkind == Z_DENSE ? "dense" : "fft", nl,
-                  nd, c->use_graph ? 1 : 0);
+    int n[4] = {0, 0, 0, 0};
+    for (auto &ln : c->launches) n[ln.kind]++;
+    std::snprintf(buf, len, "z=%s launches/iter: lineop=%d dense=%d fftwave=%d dense_mfma=%d graph=%d",
+                  c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_DENSE_MFMA],
+                  c->use_graph ? 1 : 0);
     return SCFT_OK;
 }
 

@@ -1,0 +1,206 @@
+"""Serving loop for SCFT trial solves (the phase-diagram workflow of
+README.md:59-128), on top of the libscft batch ABI.
+
+A Pool owns one batch context: slots sized from the device's free HBM and
+each slot's byte footprint (scft_slot_bytes), all slots sharing one chain
+structure.  Requests (parameters + a seed field) queue up; after every
+batched iteration, converged or failed slots are retired and waiting
+requests are admitted into the freed slots (continuous batching), so a
+request that arrives while others run starts on the next iteration.  Every
+active slot advances in the same launches; the iteration is one hipGraph
+replay while the active set is unchanged.
+
+Convergence follows the project's criterion (the reference itself runs a
+fixed ITER): ||w_new - w_old||_rms < tol over the active fields.
+"""
+import collections
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+import abi
+
+SEED_DROPLET = "droplet"
+SEED_LAMELLA = "lamella"
+SEED_FIELDS = "fields"
+SEED_FROM = "from"        # another request's converged field
+
+
+class Request:
+    __slots__ = ("rid", "params", "seed", "mask", "arrival", "slot", "admit_t", "done_t", "iters",
+                 "F", "status", "fields", "dw", "history", "keep_fields", "start_iter")
+
+    def __init__(self, rid, params, seed, mask=None, arrival=None, keep_fields=False):
+        self.rid = rid
+        self.params = params
+        self.seed = seed              # (kind, arg)
+        self.mask = mask
+        self.arrival = arrival
+        self.slot = None
+        self.admit_t = None
+        self.done_t = None
+        self.iters = 0
+        self.F = None
+        self.dw = None
+        self.status = "queued"
+        self.fields = None
+        self.history = []
+        self.keep_fields = keep_fields
+        self.start_iter = 1
+
+    @property
+    def time_to_converge(self):
+        if self.done_t is None or self.arrival is None:
+            return None
+        return self.done_t - self.arrival
+
+
+def slot_bytes(lib, params, max_nx, precision):
+    b = C.c_size_t()
+    abi.check(lib, lib.scft_slot_bytes(C.byref(params), max_nx, precision, C.byref(b)), "scft_slot_bytes")
+    return b.value
+
+
+def device_memory(lib):
+    f, t = C.c_size_t(), C.c_size_t()
+    abi.check(lib, lib.scft_device_memory(C.byref(f), C.byref(t)), "scft_device_memory")
+    return f.value, t.value
+
+
+class Pool:
+    """One batch of slots sharing a chain structure."""
+
+    def __init__(self, lib, proto, precision=abi.SCFT_FP32_SPLIT, max_slots=256, max_nx=0,
+                 hbm_fraction=0.85, tol=1e-5, max_iters=10 ** 6, record_every=0, diverge_dw=1e3):
+        self.lib = lib
+        self.proto = proto
+        self.precision = precision
+        self.tol = tol
+        self.max_iters = max_iters
+        self.record_every = record_every
+        self.diverge_dw = diverge_dw
+        self.per_slot = slot_bytes(lib, proto, max_nx, precision)
+        free, total = device_memory(lib)
+        fit = int(free * hbm_fraction // self.per_slot)
+        self.nslots = max(1, min(max_slots, fit))
+        self.batch = abi.Batch(lib, proto, self.nslots, max_nx=max_nx, precision=precision)
+        self.free_slots = collections.deque(range(self.nslots))
+        self.running = {}             # slot -> Request
+        self.queue = collections.deque()
+        self.done = []
+        self.by_id = {}
+        self.iterations = 0           # batched iterations run
+        self.slot_iterations = 0      # instance-iterations run
+        self.t_iterate = 0.0
+
+    # ------------------------------------------------------------ requests
+    def submit(self, req):
+        if req.arrival is None:
+            req.arrival = time.perf_counter()
+        self.queue.append(req)
+        self.by_id[req.rid] = req
+
+    def _seed_fields(self, req, s):
+        kind, arg = req.seed
+        import configs
+        p = req.params
+        if kind == SEED_DROPLET:
+            w, pf = configs.droplet_fields(p, **(arg or {}))
+            self.batch.set_fields(s, w, pf)
+        elif kind == SEED_LAMELLA:
+            w, pf = configs.lamella_fields(p, **(arg or {}))
+            self.batch.set_fields(s, w, pf)
+        elif kind == SEED_FIELDS:
+            w, pf = arg
+            self.batch.set_fields(s, w, pf)
+        elif kind == SEED_FROM:
+            src = self.by_id[arg]
+            if src.fields is None:
+                raise ValueError("seed request %r kept no converged fields" % (arg,))
+            self.batch.set_fields(s, *src.fields)
+        else:
+            raise ValueError("unknown seed kind %r" % (kind,))
+        if p.mask:
+            import configs as cf
+            self.batch.set_mask(s, req.mask if req.mask is not None else cf.brush_mask(p))
+
+    def _admit(self):
+        while self.queue and self.free_slots:
+            req = self.queue[0]
+            if req.seed[0] == SEED_FROM and self.by_id[req.seed[1]].status in ("queued", "running"):
+                # its seed has not converged yet: keep order, try later
+                break
+            self.queue.popleft()
+            s = self.free_slots.popleft()
+            self.batch.init_slot(s, req.params)
+            self._seed_fields(req, s)
+            self.batch.activate(s, True)
+            req.slot = s
+            req.admit_t = time.perf_counter()
+            req.status = "running"
+            self.running[s] = req
+
+    def _retire(self, s, status, out, now):
+        req = self.running.pop(s)
+        req.status = status
+        req.done_t = now
+        req.F = out.free_energy
+        req.dw = out.dw_rms
+        if req.keep_fields and status == "converged":
+            req.fields = self.batch.get_fields(s)
+        self.batch.activate(s, False)
+        self.free_slots.append(s)
+        req.slot = None
+        self.done.append(req)
+
+    # ---------------------------------------------------------- iteration
+    def step(self):
+        """Admit, run one batched iteration, retire. Returns #active slots."""
+        self._admit()
+        if not self.running:
+            return 0
+        t0 = time.perf_counter()
+        outs = self.batch.iterate()
+        now = time.perf_counter()
+        self.t_iterate += now - t0
+        self.iterations += 1
+        self.slot_iterations += len(self.running)
+        for s in list(self.running):
+            req = self.running[s]
+            req.iters += 1
+            o = outs[s]
+            if self.record_every and req.iters % self.record_every == 0:
+                req.history.append((req.iters, o.free_energy, o.dw_rms))
+            if not math.isfinite(o.free_energy) or not math.isfinite(o.dw_rms) or o.dw_rms > self.diverge_dw:
+                self._retire(s, "failed", o, now)
+            elif o.dw_rms < self.tol:
+                self._retire(s, "converged", o, now)
+            elif req.iters >= self.max_iters:
+                self._retire(s, "max_iters", o, now)
+        return len(self.running)
+
+    def busy(self):
+        return bool(self.running or self.queue)
+
+    def run(self, until=None, deadline=None):
+        """Step until no work is left (or `until()` is true, or the deadline)."""
+        while self.busy():
+            if until is not None and until():
+                break
+            if deadline is not None and time.perf_counter() > deadline:
+                break
+            if self.step() == 0 and self.queue:
+                # queue head waits on a seed that cannot run: nothing to do
+                break
+
+    def close(self):
+        self.batch.close()
+
+
+def percentiles(values, qs=(50, 99)):
+    if not values:
+        return {q: None for q in qs}
+    v = np.sort(np.asarray(values, float))
+    return {q: float(np.percentile(v, q)) for q in qs}

@@ -148,3 +148,60 @@ def test_z_operator_sizes_match_oracle(abi, configs, orc, lib, zkind, nz, mask):
         ctx.close()
     finally:
         del os.environ["SCFT_ZKIND"]
+
+
+@pytest.mark.parametrize("variant", ["wave", "lds"])
+@pytest.mark.parametrize("nx,ny", [(16, 1), (48, 1), (80, 1), (96, 1), (160, 1), (192, 1), (320, 1), (384, 1),
+                                   (512, 1), (12, 1), (32, 16), (16, 48), (64, 64)])
+def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, ny):
+    """x and y axes through the wave-per-line FFT (compile-time lengths) and
+    the generic LDS FFT, against the oracle, on a thin NZ = 6 film."""
+    os.environ["SCFT_XFFT"] = variant
+    try:
+        p = configs.chem_droplet(lx=nx * 0.25, ly=(ny * 0.25 if ny > 1 else 1.0), lz=1.5, dr=0.25, ns=11,
+                                 vol_dbk=0.2, vol_s1=0.05, len_s1=0.3)
+        ctx = abi.Context(lib, p, abi.SCFT_FP64)
+        assert ctx.shape == (nx, ny, 6)
+        sched = ctx.schedule()
+        ref = orc.state(p)
+        d = ctx.derived
+        act = [abi.FIELD_NAMES[f] for f in range(13) if d.active[f]]
+        w, pf = configs.random_fields(p, seed=6, amp=0.5, fields=act)
+        for s in (ctx, ref):
+            s.set_fields(w, pf)
+            s.set_mask(configs.brush_mask(p))
+        for it in (2, 3):
+            a, o = ctx.iterate(it), ref.iterate(it)
+            assert _srel(a.free_energy, o.free_energy) <= TOL, (a.free_energy, o.free_energy, sched)
+        wa, _ = ctx.get_fields()
+        wb, _ = ref.get_fields()
+        assert _rel(wa, wb) <= TOL
+        ctx.close()
+    finally:
+        del os.environ["SCFT_XFFT"]
+
+
+@pytest.mark.parametrize("nz,mask", [(128, 1), (127, 1), (48, 1), (64, 0), (20, 1)])
+def test_dense_valu_variant_matches_oracle(abi, configs, orc, lib, nz, mask):
+    os.environ["SCFT_ZKIND"] = "dense"
+    os.environ["SCFT_DENSE"] = "valu"
+    try:
+        p = configs.chem_droplet(lx=0.5, ly=1.0, lz=nz * 0.125, dr=0.125, ns=21, vol_dbk=0.2, vol_s1=0.05,
+                                 len_s1=0.15)
+        p.mask = mask
+        ctx = abi.Context(lib, p, abi.SCFT_FP64)
+        ref = orc.state(p)
+        d = ctx.derived
+        act = [abi.FIELD_NAMES[f] for f in range(13) if d.active[f]]
+        w, pf = configs.random_fields(p, seed=5, amp=0.5, fields=act)
+        for s in (ctx, ref):
+            s.set_fields(w, pf)
+            if p.mask:
+                s.set_mask(configs.brush_mask(p))
+        for it in (2, 3):
+            a, o = ctx.iterate(it), ref.iterate(it)
+            assert _srel(a.free_energy, o.free_energy) <= TOL, (a.free_energy, o.free_energy)
+        ctx.close()
+    finally:
+        del os.environ["SCFT_ZKIND"]
+        del os.environ["SCFT_DENSE"]
