@@ -30,13 +30,30 @@ constexpr int kRedBlocks = 256; // partial sums per reduction and instance
 constexpr int kDenseTM = 64;    // rows per k_zdense workgroup
 constexpr int kDenseHMax = 64;  // k_zdense handles NZ <= 128
 
+// Global-memory accessors.  Pointers reach the kernels inside argument
+// blocks, so the compiler cannot tell their address space and would emit
+// flat accesses, which also count against the LDS/scalar wait counter and
+// serialise every load with the LDS traffic around it.
+template <typename T>
+__device__ __forceinline__ T ldg(const T *p, size_t i)
+{
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ void stg(T *p, size_t i, T v)
+{
+    ((__attribute__((address_space(1))) T *)p)[i] = v;
+}
+
 // ------------------------------------------------------------ FFT line op
 
 struct LineOp {
     const void *src;
     void *dst;
     const double *epre;  // Boltzmann factor applied on load (first axis), or null
-    const double *epost; // Boltzmann factor applied on store (last axis), or null
+    const double *epre2; // second factor on load (see pre_factor), or null
+    const double *epost; // Boltzmann factor applied on store, or null
     const double *K;     // length L
     const cpx<double> *tw;
     int n, L, dirichlet;
@@ -50,6 +67,17 @@ struct LineOp {
 __device__ __forceinline__ int line_base(const LineOp &op, int l)
 {
     return (l / op.c1) * op.s1 + (l % op.c1) * op.s0;
+}
+
+// Factor applied on load by the first axis of a contour step.  Slices are
+// stored without their post-step Boltzmann factor, so the first axis of the
+// next step applies E_f(t) E_f(t-1): one load squared inside a block
+// (e2 == e1), two loads at a block boundary, E alone on the start slice.
+__device__ __forceinline__ double pre_factor(const double *e1, const double *e2, size_t a)
+{
+    const double x = e1[a];
+    if (!e2) return x;
+    return (e2 == e1) ? x * x : x * e2[a];
 }
 
 // grid: (line groups, ops); dynamic LDS: 2 * P * Lmax * 16 bytes
@@ -86,7 +114,7 @@ __global__ void __launch_bounds__(kThreads) k_lineop(const LineOp *__restrict__ 
         if (r < nl) {
             const int a = line_base(op, g0 + r) + t * op.stride;
             v = T(src[a]);
-            if (op.epre) v *= op.epre[a];
+            if (op.epre) v *= pre_factor(op.epre, op.epre2, a);
         }
         const int c = r >> 1;
         T *slot = reinterpret_cast<T *>(&A[c * L + (op.dirichlet ? t + 1 : t)]) + (r & 1);
@@ -126,7 +154,7 @@ __global__ void __launch_bounds__(kThreads) k_lineop(const LineOp *__restrict__ 
 struct DenseOp {
     const void *src;
     void *dst;
-    const double *epre, *epost;
+    const double *epre, *epre2, *epost;
     const double *MsT;   // [hs][ldm]: MsT[j][k] = Ms[k][j]
     const double *MaT;   // [ha][ldm]
     int n, hs, ha, ldm;
@@ -156,10 +184,10 @@ __global__ void __launch_bounds__(kThreads) k_zdense(const DenseOp *__restrict__
         if (row < op.nrows) {
             const size_t base = (size_t)row * n;
             double x1 = double(src[base + j]);
-            if (op.epre) x1 *= op.epre[base + j];
+            if (op.epre) x1 *= pre_factor(op.epre, op.epre2, base + j);
             if (j < ha) {
                 double x2 = double(src[base + n - 1 - j]);
-                if (op.epre) x2 *= op.epre[base + n - 1 - j];
+                if (op.epre) x2 *= pre_factor(op.epre, op.epre2, base + n - 1 - j);
                 s = x1 + x2;
                 a = x1 - x2;
             } else {
@@ -232,8 +260,14 @@ __global__ void __launch_bounds__(kThreads) k_zdense(const DenseOp *__restrict__
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kMfmaWaves = 8;
-constexpr int kMfmaRows = 16 * kMfmaWaves;   // rows per workgroup
+// The wave owns its line: LDS operations of one wavefront are performed in
+// order, so a wavefront-scope fence (a compiler barrier) orders the passes.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // LDS row stride of the staged Ms^T / Ma^T: a half-wave reads fragment rows
 // t and NT + t; their offset NT*LDM doubles must be 16 (mod 32) doubles so
@@ -259,117 +293,186 @@ __device__ __forceinline__ void load4(const Tin *p, double (&v)[4])
     }
 }
 
-// The K index j of the products is a summation index, so any bijection
-// between (t, l/16) and j serves if A and B use the same one: lane group
-// l/16 takes the contiguous quarter j = (l/16) NT + t, which turns each
-// lane's A-fragment loads into runs of contiguous vector loads.
+// Wave tiles of 16 rows.  A tile is loaded with coalesced row loads (lanes
+// along z), staged in LDS with the Boltzmann factors applied, and the
+// symmetric/antisymmetric A fragments are read from LDS: the MFMA operand
+// layout pairs lanes with rows, so fragments loaded straight from global
+// memory would gather 16 rows per instruction and re-fetch lines.  The
+// workgroup stages Ms^T / Ma^T once and its waves then stride over the
+// launch's tiles independently; the next tile's loads are issued before the
+// current tile's products, so one workgroup per CU keeps HBM busy.
+constexpr int kMfmaWaves = 4;
+
+template <int HSP>
+struct MfmaTile {
+    static constexpr int NC = (2 * HSP + 63) / 64;   // 64-lane chunks per row
+    static constexpr int NE = 16 * NC;               // values per lane per tile
+};
+
+// Loads of one wave tile held in registers until the tile is staged.  The
+// z operator is the first axis of a step, so epre is always set.  The
+// second factor of a block-boundary step (one step per block) is read at
+// staging time, to keep the prefetch within the registers.
+template <typename Tin, int HSP>
+struct TileRegs {
+    Tin x[MfmaTile<HSP>::NE];
+    double e[MfmaTile<HSP>::NE];
+};
+
+// Element u of a lane's tile share: row u / NC, column lane + 64 (u % NC).
+// Loads are unconditional at clamped addresses (row 0 / column 0 of the op
+// for positions past its end) so that they issue back to back; staging
+// zeroes those positions.
+template <int NC>
+__device__ __forceinline__ size_t mfma_addr(int u, int lane, int rbase, int nr, int n)
+{
+    const int r = u / NC, c = lane + 64 * (u % NC);
+    return (size_t)(rbase + (r < nr ? r : 0)) * n + (c < n ? c : 0);
+}
+
+// Tiles past the launch's end reload the last tile: the pipeline stays
+// free of branches, which would make the compiler wait for the loads early.
+template <typename Tin, int HSP>
+__device__ __forceinline__ void mfma_fetch(const DenseOp *__restrict__ ops, int tpo, int t, int ntiles, int lane,
+                                           TileRegs<Tin, HSP> &R)
+{
+    constexpr int NC = MfmaTile<HSP>::NC, NE = MfmaTile<HSP>::NE;
+    t = min(t, ntiles - 1);
+    const DenseOp &op = ops[t / tpo];
+    const int rbase = (t % tpo) * 16, n = op.n;
+    const int nr = min(16, op.nrows - rbase);
+    const Tin *src = static_cast<const Tin *>(op.src);
+#pragma unroll
+    for (int u = 0; u < NE; ++u) R.x[u] = ldg(src, mfma_addr<NC>(u, lane, rbase, nr, n));
+#pragma unroll
+    for (int u = 0; u < NE; ++u) R.e[u] = ldg(op.epre, mfma_addr<NC>(u, lane, rbase, nr, n));
+}
+
+// Stage tile t from the registers into the wave's LDS rows with the
+// Boltzmann factors applied (rows past the op's end are zero).
+template <typename Tin, int HSP, int XS>
+__device__ __forceinline__ void mfma_stage(const DenseOp *__restrict__ ops, int tpo, int t, int ntiles, int lane,
+                                           const TileRegs<Tin, HSP> &R, double *Xw)
+{
+    constexpr int NC = MfmaTile<HSP>::NC, NE = MfmaTile<HSP>::NE;
+    t = min(t, ntiles - 1);
+    const DenseOp &op = ops[t / tpo];
+    const int rbase = (t % tpo) * 16, n = op.n;
+    const int nr = min(16, op.nrows - rbase);
+    const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? 2 : 3);   // E, E^2, E E2
+    if (mode == 3) {
+        double f[NE];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) f[u] = ldg(op.epre2, mfma_addr<NC>(u, lane, rbase, nr, n));
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int r = u / NC, c = lane + 64 * (u % NC);
+            if (c < n) Xw[r * XS + c] = r < nr ? double(R.x[u]) * (R.e[u] * f[u]) : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int r = u / NC, c = lane + 64 * (u % NC);
+            const double fac = mode == 2 ? R.e[u] * R.e[u] : R.e[u];
+            if (c < n) Xw[r * XS + c] = r < nr ? double(R.x[u]) * fac : 0.0;
+        }
+    }
+}
+
+// Per wave and tile: build the A fragments of tile t from LDS, stage tile
+// t+1 (loaded during the previous tile's products) into the freed rows,
+// issue the loads of tile t+2, run the products, store D straight from the
+// accumulators.  The loads get a whole tile of products to land.
 template <typename Tin, typename Tout, int HSP>
-__global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *__restrict__ ops)
+__global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *__restrict__ ops, int ntiles, int tpo)
 {
     constexpr int NT = HSP / 4;    // K steps
     constexpr int NK = HSP / 16;   // output column tiles
     constexpr int LDM = mfma_ldm(HSP);
-    const DenseOp &op = ops[blockIdx.y];
-    const int row0 = blockIdx.x * kMfmaRows;
-    if (row0 >= op.nrows) return;
+    constexpr int XS = 2 * HSP + 1;   // staged row stride (doubles), odd: rows hit different banks
     __shared__ double Ms[HSP * LDM];
     __shared__ double Ma[HSP * LDM];
-    const int n = op.n, hs = op.hs, ha = op.ha;
+    __shared__ double X[kMfmaWaves * 16 * XS];
+    // every op of a launch shares one operator (same NZ, LZ and NS)
+    const int n = ops[0].n, hs = ops[0].hs, ha = ops[0].ha;
     const int tid = threadIdx.x;
     for (int e = tid; e < HSP * HSP; e += 64 * kMfmaWaves) {
         const int j = e / HSP, k = e - j * HSP;
-        Ms[j * LDM + k] = (j < hs && k < hs) ? op.MsT[(size_t)j * op.ldm + k] : 0.0;
-        Ma[j * LDM + k] = (j < ha && k < ha) ? op.MaT[(size_t)j * op.ldm + k] : 0.0;
-    }
-    const Tin *__restrict__ src = static_cast<const Tin *>(op.src);
-    const double *__restrict__ ep = op.epre;
-    Tout *__restrict__ dst = static_cast<Tout *>(op.dst);
-    const int lane = tid & 63, wave = tid >> 6;
-    const int li = lane & 15, lj = lane >> 4;
-    const int rbase = row0 + wave * 16;
-    double sf[NT], af[NT];
-    {
-        const int row = rbase + li;
-        const bool ok = row < op.nrows;
-        const size_t base = (size_t)(ok ? row : 0) * n;
-        const int j0 = lj * NT;
-        if (ok && hs == HSP && ha == HSP && (n % 4) == 0) {
-            // full halves, n = 2 HSP: x1 runs up from j0, x2 runs down from n-1-j0
-#pragma unroll
-            for (int t = 0; t < NT; t += 4) {
-                double x1[4], x2[4];
-                load4<Tin>(src + base + j0 + t, x1);
-                load4<Tin>(src + base + n - 4 - j0 - t, x2);   // x2[3-i] = x[n-1-(j0+t+i)]
-                if (ep) {
-                    double e1[4], e2[4];
-                    load4<double>(ep + base + j0 + t, e1);
-                    load4<double>(ep + base + n - 4 - j0 - t, e2);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) { x1[i] *= e1[i]; x2[i] *= e2[i]; }
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    sf[t + i] = x1[i] + x2[3 - i];
-                    af[t + i] = x1[i] - x2[3 - i];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const int j = j0 + t;
-                double s = 0.0, a = 0.0;
-                if (ok && j < hs) {
-                    double x1 = double(src[base + j]);
-                    if (ep) x1 *= ep[base + j];
-                    if (j < ha) {
-                        double x2 = double(src[base + n - 1 - j]);
-                        if (ep) x2 *= ep[base + n - 1 - j];
-                        s = x1 + x2;
-                        a = x1 - x2;
-                    } else {
-                        s = x1;   // middle point of an odd line
-                    }
-                }
-                sf[t] = s;
-                af[t] = a;
-            }
-        }
+        Ms[j * LDM + k] = (j < hs && k < hs) ? ldg(ops[0].MsT, (size_t)j * ops[0].ldm + k) : 0.0;
+        Ma[j * LDM + k] = (j < ha && k < ha) ? ldg(ops[0].MaT, (size_t)j * ops[0].ldm + k) : 0.0;
     }
     __syncthreads();
-    d4_t U[NK], W[NK];
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: tile indices stay scalar
+    const int li = lane & 15, lj = lane >> 4;
+    double *Xw = X + wave * 16 * XS;
+    const int wstride = gridDim.x * kMfmaWaves;
+    int t = blockIdx.x * kMfmaWaves + wave;
+    if (t >= ntiles) return;   // no workgroup barrier follows
+    TileRegs<Tin, HSP> R;
+    mfma_fetch<Tin, HSP>(ops, tpo, t, ntiles, lane, R);
+    mfma_stage<Tin, HSP, XS>(ops, tpo, t, ntiles, lane, R, Xw);
+    mfma_fetch<Tin, HSP>(ops, tpo, t + wstride, ntiles, lane, R);
+    for (; t < ntiles; t += wstride) {
+        wave_sync();   // tile t staged
+        double sf[NT], af[NT];
 #pragma unroll
-    for (int kt = 0; kt < NK; ++kt) {
-        U[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
-        W[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double *ms = Ms + (lj * NT + t) * LDM + li;
-        const double *ma = Ma + (lj * NT + t) * LDM + li;
+        for (int k = 0; k < NT; ++k) {
+            const int j = 4 * k + lj;
+            double sv = 0.0, av = 0.0;
+            if (j < ha) {
+                const double x1 = Xw[li * XS + j], x2 = Xw[li * XS + n - 1 - j];
+                sv = x1 + x2;
+                av = x1 - x2;
+            } else if (j < hs) {
+                sv = Xw[li * XS + j];   // middle point of an odd line
+            }
+            sf[k] = sv;
+            af[k] = av;
+        }
+        wave_sync();   // every lane has read its fragments
+        mfma_stage<Tin, HSP, XS>(ops, tpo, t + wstride, ntiles, lane, R, Xw);
+        mfma_fetch<Tin, HSP>(ops, tpo, t + 2 * wstride, ntiles, lane, R);
+        d4_t U[NK], W[NK];
 #pragma unroll
         for (int kt = 0; kt < NK; ++kt) {
-            U[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[t], ms[16 * kt], U[kt], 0, 0, 0);
-            W[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], ma[16 * kt], W[kt], 0, 0, 0);
+            U[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
+            W[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
         }
-    }
-    // D layout: register r of lane l is row l/16 + 4r, column l%16 of the tile
 #pragma unroll
-    for (int kt = 0; kt < NK; ++kt) {
-        const int k = 16 * kt + li;
-        if (k >= hs) continue;
+        for (int k = 0; k < NT; ++k) {
+            const double *ms = Ms + (4 * k + lj) * LDM + li;
+            const double *ma = Ma + (4 * k + lj) * LDM + li;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = rbase + lj + 4 * r;
-            if (row >= op.nrows) continue;
-            const size_t base = (size_t)row * n;
-            const double u = U[kt][r], v = W[kt][r];
-            double y1 = u + v;
-            if (op.epost) y1 *= op.epost[base + k];
-            dst[base + k] = Tout(y1);
-            if (k < ha) {
-                double y2 = u - v;
-                if (op.epost) y2 *= op.epost[base + n - 1 - k];
-                dst[base + n - 1 - k] = Tout(y2);
+            for (int kt = 0; kt < NK; ++kt) {
+                U[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[k], ms[16 * kt], U[kt], 0, 0, 0);
+                W[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[k], ma[16 * kt], W[kt], 0, 0, 0);
+            }
+        }
+        // D layout: register r of lane l is row l/16 + 4r, column l%16 of the
+        // tile: 16 lanes store 16 consecutive z points of one row
+        const DenseOp &op = ops[t / tpo];
+        const int rbase = (t % tpo) * 16;
+        const int nr = min(16, op.nrows - rbase);
+        Tout *dst = static_cast<Tout *>(op.dst);
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) {
+            const int k = 16 * kt + li;
+            if (k >= hs) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = lj + 4 * r;
+                if (rr >= nr) continue;
+                const size_t row = (size_t)(rbase + rr) * n;
+                const double u = U[kt][r], v = W[kt][r];
+                double y = u + v;
+                if (op.epost) y *= ldg(op.epost, row + k);
+                stg(dst, row + k, Tout(y));
+                if (k < ha) {
+                    double y2 = u - v;
+                    if (op.epost) y2 *= ldg(op.epost, row + n - 1 - k);
+                    stg(dst, row + n - 1 - k, Tout(y2));
+                }
             }
         }
     }
@@ -453,14 +556,6 @@ __device__ __forceinline__ void dft_small(cpx<double> (&v)[R], double sg)
     }
 }
 
-// The wave owns its line: LDS operations of one wavefront are performed in
-// order, so a wavefront-scope fence (a compiler barrier) orders the passes.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One Stockham pass of the wave's line `buf` (in place: read all, sync, write all).
 template <int L, int P, int SIGN, bool APPLY_K>
@@ -538,23 +633,52 @@ __global__ void __launch_bounds__(64 * kFftWaves) k_fftwave(const LineOp *__rest
     const Tin *__restrict__ src = static_cast<const Tin *>(op.src);
     Tout *__restrict__ dst = static_cast<Tout *>(op.dst);
     for (int e = tid; e < L; e += 64 * kFftWaves) {
-        tw[e] = op.tw[e];
-        K[e] = op.K[e];
+        const double *twd = reinterpret_cast<const double *>(op.tw);
+        tw[e] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
+        K[e] = ldg(op.K, e);
     }
     if (tid < NL) lbase[tid] = tid < nl ? line_base(op, g0 + tid) : 0;
     __syncthreads();
     const bool contiguous = (op.stride == 1);
     const int stride = op.stride;
-    for (int e = tid; e < NL * L; e += 64 * kFftWaves) {
+    // all of a thread's loads issue before the first use (lines past the
+    // op's end read line 0 and are zeroed)
+    constexpr int NTH = 64 * kFftWaves;
+    constexpr int NPT = (NL * L + NTH - 1) / NTH;
+    const int mode = !op.epre ? 0 : (!op.epre2 ? 1 : (op.epre2 == op.epre ? 2 : 3));
+    int ad[NPT];
+    Tin xv[NPT];
+    double ev[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int e = tid + i * NTH;
+        int r, t;
+        if (contiguous) { r = e / L; t = e - r * L; }
+        else { t = e / NL; r = e - t * NL; }
+        ad[i] = (e < NL * L && r < nl) ? lbase[r] + t * stride : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) xv[i] = ldg(src, ad[i]);
+    if (mode) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) ev[i] = ldg(op.epre, ad[i]);
+        if (mode == 2) {
+#pragma unroll
+            for (int i = 0; i < NPT; ++i) ev[i] *= ev[i];
+        } else if (mode == 3) {
+#pragma unroll
+            for (int i = 0; i < NPT; ++i) ev[i] *= ldg(op.epre2, ad[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int e = tid + i * NTH;
+        if (e >= NL * L) break;
         int r, t;
         if (contiguous) { r = e / L; t = e - r * L; }
         else { t = e / NL; r = e - t * NL; }
         double v = 0.0;
-        if (r < nl) {
-            const int a = lbase[r] + t * stride;
-            v = double(src[a]);
-            if (op.epre) v *= op.epre[a];
-        }
+        if (r < nl) v = mode ? double(xv[i]) * ev[i] : double(xv[i]);
         reinterpret_cast<double *>(&buf[(r >> 1) * LS + t])[r & 1] = v;
     }
     __syncthreads();
@@ -570,8 +694,8 @@ __global__ void __launch_bounds__(64 * kFftWaves) k_fftwave(const LineOp *__rest
         if (r >= nl) continue;
         const int a = lbase[r] + t * stride;
         double v = reinterpret_cast<const double *>(&buf[(r >> 1) * LS + t])[r & 1] * scale;
-        if (op.epost) v *= op.epost[a];
-        dst[a] = Tout(v);
+        if (op.epost) v *= ldg(op.epost, a);
+        stg(dst, a, Tout(v));
     }
 }
 
@@ -654,7 +778,8 @@ struct Scalars {
 };
 
 struct QSlot {
-    const void *end;   // q(NS_p - 1) slice (T)
+    const void *end;   // q(NS_p - 1) slice (T), stored without its factor
+    const double *e;   // Boltzmann factor owed by that slice, or null
     double *partial;   // [nblocks]
     int V;
 };
@@ -667,16 +792,20 @@ __global__ void __launch_bounds__(kThreads) k_qsum(const QSlot *__restrict__ slo
     const T *q = static_cast<const T *>(s.end);
     double v[1] = {0.0};
     for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < s.V; idx += gridDim.x * kThreads)
-        v[0] += double(q[idx]);
+        v[0] += s.e ? double(q[idx]) * s.e[idx] : double(q[idx]);
     block_sum<1>(v);
     if (threadIdx.x == 0) s.partial[blockIdx.x] = v[0];
 }
 
 // ------------------------------------------------ densities + F + RMSDev
 
+// Slices are stored without the factor of the step that made them: slice s
+// of a chain owes E_a for 1 <= s <= switch_at, E_b above, nothing at s = 0.
 struct DenTerm {
     const void *q;
     const void *qd;
+    const double *eqa, *eqb, *eda, *edb;   // factors of the q and q-dagger chains
+    int qsw, dsw;                          // their switch slices
     int ns, start, stop;
     int field;   // 0..12
     int sp;      // species (index into Scalars::Q)
@@ -706,7 +835,13 @@ __device__ __forceinline__ double simpson(const DenTerm &t, int V, int idx)
     const int top = t.ns - 1;
     const T *q = static_cast<const T *>(t.q);
     const T *qd = static_cast<const T *>(t.qd);
-    auto qc = [&](int s) -> double { return double(q[(size_t)s * sV + idx]) * double(qd[(size_t)(top - s) * sV + idx]); };
+    const double qa = t.eqa[idx], qb = t.eqb[idx], da = t.eda[idx], db = t.edb[idx];
+    auto fac = [](int s, int sw, double a, double b) { return s == 0 ? 1.0 : (s <= sw ? a : b); };
+    auto qc = [&](int s) -> double {
+        const int u = top - s;
+        return (double(q[(size_t)s * sV + idx]) * fac(s, t.qsw, qa, qb)) *
+               (double(qd[(size_t)u * sV + idx]) * fac(u, t.dsw, da, db));
+    };
     if constexpr (sizeof(T) == 8) {
         // the reference's order: ends, then 4*odd, then 2*even (SCFT.c:2283-2297)
         double phi = qc(t.start) + qc(t.stop);
@@ -914,9 +1049,10 @@ __global__ void __launch_bounds__(kThreads) k_convert(const double *src, T *dst,
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_widen(const T *src, double *dst, int n)
+__global__ void __launch_bounds__(kThreads) k_widen(const T *src, const double *e, double *dst, int n)
 {
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) dst[i] = double(src[i]);
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+        dst[i] = e ? double(src[i]) * e[i] : double(src[i]);
 }
 
 template <typename T>

@@ -165,10 +165,17 @@ std::map<std::tuple<int, int, long long, int>, int> g_tune;
 
 // ---------------------------------------------------------------- context
 
+// A propagator chain: step t (slice t -> t+1) uses field fa while
+// t < switch_at, fb afterwards.  Slices are stored without the Boltzmann
+// factor of the step that produced them (q(s) = E(s) * stored(s), E(0) = 1);
+// the first axis of the next step applies it together with its own.
 struct Chain {
     void *base;       // q or q-dagger array of the species (T)
-    double *tmp;      // V doubles between axis operators
+    void *tmp;        // V values between axis operators (T, or FP64 when wide)
     int nsteps, fa, fb, switch_at;
+    int step_field(int t) const { return t < switch_at ? fa : fb; }
+    // field of the factor owed by stored slice s, or -1 (slice 0)
+    int slice_field(int s) const { return s == 0 ? -1 : step_field(s - 1); }
 };
 
 struct Slot {
@@ -227,6 +234,8 @@ struct scft_ctx {
     int zkind = Z_FFT;
     bool fft_wave = true;            // wave-per-line FFT for supported periodic lengths
     bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
+    bool tmp_wide = false;           // FP64 scratch between axis operators in FP32 mode
+    int ncu = 256;                   // compute units of the device (persistent launches)
     Scalars *scal = nullptr;         // [nslots] device
     Scalars *scal_host = nullptr;    // [nslots] pinned
     // schedule of the active set
@@ -404,8 +413,26 @@ int slot_alloc(scft_ctx *c, Slot &s)
         add(s.q[sp], d.ns_sp[sp] - 1, 10 + b, 10 + b, 1 << 30);   // from the mask (SCFT.c:2441)
         add(s.qd[sp], d.ns_sp[sp] - 1, 10 + b, 10 + b, 1 << 30);
     }
-    for (auto &ch : s.chains) RETCHK(ctx_alloc(c, &ch.tmp, V));
+    for (auto &ch : s.chains) {
+        double *t = nullptr;
+        RETCHK(ctx_alloc(c, &t, V));
+        ch.tmp = t;
+    }
     return SCFT_OK;
+}
+
+const Chain *find_chain(const Slot &s, const void *base)
+{
+    for (auto &ch : s.chains)
+        if (ch.base == base) return &ch;
+    return nullptr;
+}
+
+// Boltzmann factor owed by stored slice idx of a chain (null for slice 0).
+const double *slice_factor(const Slot &s, const Chain &ch, int idx)
+{
+    const int f = ch.slice_field(idx);
+    return f < 0 ? nullptr : s.E + (size_t)f * s.V;
 }
 
 // (Re)initialise a slot for parameter set p: tables, argument blocks, zero
@@ -501,8 +528,14 @@ int slot_init(scft_ctx *c, Slot &s, const scft_params &p)
         const int ns = d.ns_sp[sp];
         if (ns <= 0) continue;
         const void *src = (sp < 2) ? s.qd[sp] : s.q[sp];
+        const Chain *ch = find_chain(s, src);
+        if (!ch) {
+            set_err("internal: no chain for a Q slice");
+            return SCFT_EINVAL;
+        }
         QSlot qs;
         qs.end = static_cast<const char *>(src) + (size_t)(ns - 1) * V * c->tsize;
+        qs.e = slice_factor(s, *ch, ns - 1);
         qs.partial = s.qpart + (size_t)fin.nq * c->nred;   // k_qfinal reads [nq][nred]
         qs.V = s.V;
         s.qslots.push_back(qs);
@@ -536,6 +569,13 @@ int slot_init(scft_ctx *c, Slot &s, const scft_params &p)
         t.field = field;
         t.sp = sp;
         t.frac = frac;
+        const Chain *cq = find_chain(s, t.q), *cd = find_chain(s, t.qd);
+        t.eqa = s.E + (size_t)cq->fa * V;
+        t.eqb = s.E + (size_t)cq->fb * V;
+        t.qsw = cq->switch_at;
+        t.eda = s.E + (size_t)cd->fa * V;
+        t.edb = s.E + (size_t)cd->fb * V;
+        t.dsw = cd->switch_at;
     };
     const double cv = d.coating_vol;
     for (int h = 0; h < 10; ++h)
@@ -648,16 +688,18 @@ int build_schedule(scft_ctx *c)
                 const size_t V = s.V;
                 for (auto &ch : s.chains) {
                     if (ch.nsteps <= t) continue;
-                    const int f = (t < ch.switch_at) ? ch.fa : ch.fb;
                     const void *src = (ai == 0) ? static_cast<const char *>(ch.base) + (size_t)t * V * ts : (const void *)ch.tmp;
                     void *dst = (ai == nax - 1) ? static_cast<char *>(ch.base) + (size_t)(t + 1) * V * ts : (void *)ch.tmp;
-                    const double *ep = s.E + (size_t)f * V;
+                    // first axis: E of this step times the factor the stored slice owes
+                    const double *ep = (ai == 0) ? s.E + (size_t)ch.step_field(t) * V : nullptr;
+                    const double *ep2 = (ai == 0) ? slice_factor(s, ch, t) : nullptr;
                     if (dense) {
                         DenseOp op{};
                         op.src = src;
                         op.dst = dst;
-                        op.epre = (ai == 0) ? ep : nullptr;
-                        op.epost = (ai == nax - 1) ? ep : nullptr;
+                        op.epre = ep;
+                        op.epre2 = ep2;
+                        op.epost = nullptr;
                         op.MsT = s.dz->MsT;
                         op.MaT = s.dz->MaT;
                         op.n = s.dz->n;
@@ -672,8 +714,9 @@ int build_schedule(scft_ctx *c)
                     LineOp op{};
                     op.src = src;
                     op.dst = dst;
-                    op.epre = (ai == 0) ? ep : nullptr;
-                    op.epost = (ai == nax - 1) ? ep : nullptr;
+                    op.epre = ep;
+                    op.epre2 = ep2;
+                    op.epost = nullptr;
                     op.K = ft->K;
                     op.tw = ft->tw;
                     op.L = ft->L;
@@ -704,10 +747,15 @@ int build_schedule(scft_ctx *c)
                 int maxl = 0;
                 for (auto &op : td) maxl = std::max(maxl, op.nrows);
                 const int hsp = ((td[0].hs + 15) / 16) * 16;
-                if (c->dense_mfma && hsp <= 64) {
+                bool one_table = true;   // the MFMA kernel also needs E on load (first axis)
+                for (auto &op : td) one_table &= op.MsT == td[0].MsT && op.n == td[0].n && op.epre;
+                if (c->dense_mfma && hsp <= 64 && one_table) {
+                    // persistent waves over (op, 16-row tile); one workgroup per CU
                     ln.kind = LK_DENSE_MFMA;
                     ln.hsp = hsp;
-                    ln.gx = (maxl + kMfmaRows - 1) / kMfmaRows;
+                    ln.P = (maxl + 15) / 16;   // tiles per op
+                    const int waves = ln.count * ln.P;
+                    ln.gx = std::max(1, std::min((waves + kMfmaWaves - 1) / kMfmaWaves, c->ncu));
                 } else {
                     ln.kind = LK_DENSE;
                     ln.gx = (maxl + kDenseTM - 1) / kDenseTM;
@@ -816,8 +864,8 @@ void launch_prep(scft_ctx *c, int nact, int maxV)
                        (const PrepArgs *)c->d_prep);
 }
 
-// Typed launchers: the axis position fixes the storage types (first axis
-// reads the T slice, the last writes the T slice, the rest use FP64 tmp).
+// Typed launchers: slices are T; the scratch between axes is T too, or FP64
+// when tmp_wide (then the axis position fixes the input/output types).
 template <typename Tin, typename Tout>
 void launch_fftwave(scft_ctx *c, const Launch &ln)
 {
@@ -843,11 +891,12 @@ void launch_one(scft_ctx *c, const Launch &ln)
     switch (ln.kind) {
     case LK_DENSE_MFMA: {
         const DenseOp *ops = c->denseops + ln.first;
-        const dim3 b(64 * kMfmaWaves);
-        if (ln.hsp == 16) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 16>), g, b, 0, st, ops);
-        else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 32>), g, b, 0, st, ops);
-        else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 48>), g, b, 0, st, ops);
-        else hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 64>), g, b, 0, st, ops);
+        const dim3 g1(ln.gx), b(64 * kMfmaWaves);
+        const int nt = ln.count * ln.P;
+        if (ln.hsp == 16) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 16>), g1, b, 0, st, ops, nt, ln.P);
+        else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 32>), g1, b, 0, st, ops, nt, ln.P);
+        else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 48>), g1, b, 0, st, ops, nt, ln.P);
+        else hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 64>), g1, b, 0, st, ops, nt, ln.P);
         break;
     }
     case LK_DENSE:
@@ -866,6 +915,10 @@ template <typename T>
 void launch_steps(scft_ctx *c)
 {
     for (const Launch &ln : c->launches) {
+        if (!c->tmp_wide) {
+            launch_one<T, T>(c, ln);
+            continue;
+        }
         switch (ln.pos) {
         case 0: launch_one<T, double>(c, ln); break;
         case 1: launch_one<double, double>(c, ln); break;
@@ -1095,11 +1148,20 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     const char *g = std::getenv("SCFT_GRAPH");
     c->use_graph = !(g && g[0] == '0');
     // kernel variants (tests force each one): SCFT_XFFT=lds selects the
-    // generic LDS FFT for periodic axes, SCFT_DENSE=valu the VALU dense z op
+    // generic LDS FFT for periodic axes, SCFT_DENSE=valu the VALU dense z op,
+    // SCFT_TMP=fp64 FP64 scratch between axis operators in FP32 mode
     const char *xf = std::getenv("SCFT_XFFT");
     c->fft_wave = !(xf && std::strcmp(xf, "lds") == 0);
     const char *dm = std::getenv("SCFT_DENSE");
     c->dense_mfma = !(dm && std::strcmp(dm, "valu") == 0);
+    const char *tw = std::getenv("SCFT_TMP");
+    c->tmp_wide = tw && std::strcmp(tw, "fp64") == 0;
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            c->ncu = ncu;
+    }
     auto fail = [&](int r) {
         free_ctx(c);
         return r;
@@ -1288,13 +1350,20 @@ int scft_slot_get_propagator(scft_ctx *c, int slot, int sp, int which, int idx, 
         set_err("scft_get_propagator: no such slice");
         return SCFT_EINVAL;
     }
-    const char *src = static_cast<const char *>(which ? s->qd[sp] : s->q[sp]) + (size_t)idx * s->V * c->tsize;
+    const void *base = which ? s->qd[sp] : s->q[sp];
+    const Chain *ch = find_chain(*s, base);
+    if (!ch) {
+        set_err("scft_get_propagator: no chain for the slice");
+        return SCFT_EINVAL;
+    }
+    const double *fac = slice_factor(*s, *ch, idx);
+    const char *src = static_cast<const char *>(base) + (size_t)idx * s->V * c->tsize;
     double *tmp = nullptr;
     RETCHK(dev_alloc(&tmp, s->V));
     if (c->tsize == 8)
-        hipLaunchKernelGGL(k_widen<double>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream, (const double *)src, tmp, s->V);
+        hipLaunchKernelGGL(k_widen<double>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream, (const double *)src, fac, tmp, s->V);
     else
-        hipLaunchKernelGGL(k_widen<float>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream, (const float *)src, tmp, s->V);
+        hipLaunchKernelGGL(k_widen<float>, dim3(grid_for(s->V)), dim3(kThreads), 0, c->stream, (const float *)src, fac, tmp, s->V);
     hipError_t e = hipMemcpyAsync(out, tmp, sizeof(double) * s->V, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
@@ -1422,15 +1491,20 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
         if ((kind == 0) != is_z) continue;
         if (kind == 1 && ln.pos != 2 && ln.pos != 3) continue;
         sel.push_back(ln);
-        // algorithmic traffic: read src + E, write dst (T at the slice end, FP64 scratch inside)
-        const double in_b = (ln.pos == 0 || ln.pos == 3) ? ts : 8.0;
-        const double out_b = (ln.pos == 2 || ln.pos == 3) ? ts : 8.0;
+        // algorithmic traffic: read src and the Boltzmann factors (first axis:
+        // one FP64 array, two at a block boundary), write dst; slices are T,
+        // the scratch between axes T (FP64 when wide)
+        const double in_b = (ln.pos == 0 || ln.pos == 3 || !c->tmp_wide) ? ts : 8.0;
+        const double out_b = (ln.pos == 2 || ln.pos == 3 || !c->tmp_wide) ? ts : 8.0;
+        auto e_bytes = [](const double *e1, const double *e2, const double *ep) {
+            return (e1 ? (e2 && e2 != e1 ? 16.0 : 8.0) : 0.0) + (ep ? 8.0 : 0.0);
+        };
         if (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA) {
             std::vector<DenseOp> ops(ln.count);
             HIPCHK(hipMemcpy(ops.data(), c->denseops + ln.first, ln.count * sizeof(DenseOp), hipMemcpyDeviceToHost));
             for (auto &op : ops) {
                 const double elems = double(op.nrows) * op.n;
-                B += elems * (in_b + out_b + 8.0);
+                B += elems * (in_b + out_b + e_bytes(op.epre, op.epre2, op.epost));
                 Fl += 2.0 * op.nrows * (double(op.hs) * op.hs + double(op.ha) * op.ha);
             }
         } else {
@@ -1438,7 +1512,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
             HIPCHK(hipMemcpy(ops.data(), c->lineops + ln.first, ln.count * sizeof(LineOp), hipMemcpyDeviceToHost));
             for (auto &op : ops) {
                 const double elems = double(op.nlines) * op.n;
-                B += elems * (in_b + out_b + 8.0);
+                B += elems * (in_b + out_b + e_bytes(op.epre, op.epre2, op.epost));
                 // radix-2-equivalent count of a complex transform pair: 2 x 5 L log2 L per packed line pair
                 Fl += (op.nlines / 2.0) * 2.0 * 5.0 * op.L * std::log2(double(op.L));
             }
