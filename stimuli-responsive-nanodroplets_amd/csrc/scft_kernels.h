@@ -46,6 +46,28 @@ __device__ __forceinline__ void stg(T *p, size_t i, T v)
     ((__attribute__((address_space(1))) T *)p)[i] = v;
 }
 
+// p[i], p[i + 1] in one store (i even, p aligned to two elements).
+template <typename T>
+__device__ __forceinline__ void stg_pair(T *p, size_t i, T a, T b)
+{
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    ((__attribute__((address_space(1))) v2 *)p)[i / 2] = v2{a, b};
+}
+
+// p[i], p[i + 1] in one load (i even, p aligned to two elements).
+template <typename T>
+struct Pair {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ Pair<T> ldg_pair(const T *p, size_t i)
+{
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = ((const __attribute__((address_space(1))) v2 *)p)[i / 2];
+    return Pair<T>{v.x, v.y};
+}
+
 // ------------------------------------------------------------ FFT line op
 
 struct LineOp {
@@ -473,6 +495,130 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *
                     if (op.epost) y2 *= ldg(op.epost, row + n - 1 - k);
                     stg(dst, row + n - 1 - k, Tout(y2));
                 }
+            }
+        }
+    }
+}
+
+// Even NZ (hs == ha): two wavefronts per SIMD.  The staging rows are
+// halved (8 at a time) so that eight waves and Ms^T / Ma^T fit in LDS; each
+// lane loads adjacent point pairs, rows past the op's end load a valid row
+// and are never stored, so the staging needs no masks.  The K index is
+// permuted (j = lj NT + k): a lane's fragment values are consecutive z
+// points and the two fragment rows a half-wave reads from Ms^T / Ma^T fall
+// into disjoint bank halves (mfma_ldm).
+constexpr int kMfma2Waves = 8;
+
+template <int HSP>
+struct Mfma2 {
+    static constexpr int NT = HSP / 4, NK = HSP / 16, LDM = mfma_ldm(HSP);
+    static constexpr int XS = 2 * HSP + 2;   // even: rows stay 16-byte aligned
+};
+
+template <typename Tin, typename Tout, int HSP>
+__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp *__restrict__ ops, int ntiles, int tpo)
+{
+    constexpr int NT = Mfma2<HSP>::NT, NK = Mfma2<HSP>::NK, LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
+    __shared__ double Ms[HSP * LDM];
+    __shared__ double Ma[HSP * LDM];
+    __shared__ double X[kMfma2Waves * 8 * XS];
+    const int n = ops[0].n, hs = ops[0].hs;   // n = 2 hs
+    const int tid = threadIdx.x;
+    for (int e = tid; e < HSP * HSP; e += 64 * kMfma2Waves) {
+        const int j = e / HSP, k = e - j * HSP;
+        const bool in = j < hs && k < hs;
+        Ms[j * LDM + k] = in ? ldg(ops[0].MsT, (size_t)j * ops[0].ldm + k) : 0.0;
+        Ma[j * LDM + k] = in ? ldg(ops[0].MaT, (size_t)j * ops[0].ldm + k) : 0.0;
+    }
+    __syncthreads();
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: tile indices stay scalar
+    const int li = lane & 15, lj = lane >> 4;
+    double *Xw = X + wave * 8 * XS;
+    const int c = 2 * lane;                 // this lane's point pair in a row
+    const bool cl = c < 2 * HSP;            // lanes that own staged columns
+    const int cc = c + 1 < n ? c : 0;       // load column (pairs past the row read column 0)
+    for (int t = blockIdx.x * kMfma2Waves + wave; t < ntiles; t += gridDim.x * kMfma2Waves) {
+        const DenseOp &op = ops[t / tpo];
+        const int rbase = (t % tpo) * 16;
+        const int nr = min(16, op.nrows - rbase);
+        const Tin *src = static_cast<const Tin *>(op.src);
+        // factor: E alone (start slice), E^2 inside a block, E E2 at a block boundary
+        const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? 2 : 3);
+        Pair<Tin> xa[16];
+        Pair<double> ea[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const size_t a = (size_t)(rbase + (r < nr ? r : 0)) * n + cc;
+            xa[r] = ldg_pair(src, a);
+            ea[r] = ldg_pair(op.epre, a);
+        }
+        double sf[NT], af[NT];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            wave_sync();   // the previous reads of the staging rows are done
+            if (cl) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const int R = 8 * h + r;
+                    double e0 = ea[R].x, e1 = ea[R].y;
+                    if (mode == 2) {
+                        e0 *= e0;
+                        e1 *= e1;
+                    } else if (mode == 3) {   // one step per block: load the second factor here
+                        const Pair<double> f = ldg_pair(op.epre2, (size_t)(rbase + (R < nr ? R : 0)) * n + cc);
+                        e0 *= f.x;
+                        e1 *= f.y;
+                    }
+                    double2 v;
+                    v.x = double(xa[R].x) * e0;
+                    v.y = double(xa[R].y) * e1;
+                    *reinterpret_cast<double2 *>(&Xw[r * XS + c]) = v;
+                }
+            }
+            wave_sync();
+            if ((li >> 3) == h) {
+                const double *row = Xw + (li & 7) * XS;
+#pragma unroll
+                for (int k = 0; k < NT; ++k) {
+                    const int j = lj * NT + k;   // j >= hs meets zero rows of Ms^T / Ma^T
+                    const double x1 = row[j], x2 = row[max(n - 1 - j, 0)];
+                    sf[k] = x1 + x2;
+                    af[k] = x1 - x2;
+                }
+            }
+        }
+        d4_t U[NK], W[NK];
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) {
+            U[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
+            W[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
+        }
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const double *ms = Ms + (lj * NT + k) * LDM + li;
+            const double *ma = Ma + (lj * NT + k) * LDM + li;
+#pragma unroll
+            for (int kt = 0; kt < NK; ++kt) {
+                U[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[k], ms[16 * kt], U[kt], 0, 0, 0);
+                W[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[k], ma[16 * kt], W[kt], 0, 0, 0);
+            }
+        }
+        // D layout: register r of lane l is row l/16 + 4r, output point l%16
+        // of the column tile; the mirrored half goes to n-1-k
+        Tout *dst = static_cast<Tout *>(op.dst);
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) {
+            const int k = 16 * kt + li;
+            if (k >= hs) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = lj + 4 * r;
+                if (rr >= nr) continue;
+                const size_t row = (size_t)(rbase + rr) * n;
+                const double u = U[kt][r], v = W[kt][r];
+                stg(dst, row + k, Tout(u + v));
+                stg(dst, row + n - 1 - k, Tout(u - v));
             }
         }
     }

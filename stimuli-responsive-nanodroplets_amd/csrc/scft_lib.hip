@@ -199,7 +199,7 @@ struct Slot {
     std::vector<QSlot> qslots;
 };
 
-enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3 };
+enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4 };
 
 struct Launch {
     int kind;        // LaunchKind
@@ -234,6 +234,7 @@ struct scft_ctx {
     int zkind = Z_FFT;
     bool fft_wave = true;            // wave-per-line FFT for supported periodic lengths
     bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
+    bool dense_mfma1 = false;        // force the one-wave-per-SIMD MFMA kernel for even NZ too
     bool tmp_wide = false;           // FP64 scratch between axis operators in FP32 mode
     int ncu = 256;                   // compute units of the device (persistent launches)
     Scalars *scal = nullptr;         // [nslots] device
@@ -747,15 +748,18 @@ int build_schedule(scft_ctx *c)
                 int maxl = 0;
                 for (auto &op : td) maxl = std::max(maxl, op.nrows);
                 const int hsp = ((td[0].hs + 15) / 16) * 16;
-                bool one_table = true;   // the MFMA kernel also needs E on load (first axis)
-                for (auto &op : td) one_table &= op.MsT == td[0].MsT && op.n == td[0].n && op.epre;
+                // the MFMA kernels take E on load (first axis) and no factor on store
+                bool one_table = true;
+                for (auto &op : td) one_table &= op.MsT == td[0].MsT && op.n == td[0].n && op.epre && !op.epost;
                 if (c->dense_mfma && hsp <= 64 && one_table) {
                     // persistent waves over (op, 16-row tile); one workgroup per CU
-                    ln.kind = LK_DENSE_MFMA;
+                    const bool even = td[0].hs == td[0].ha && !c->dense_mfma1;
+                    const int wpg = even ? kMfma2Waves : kMfmaWaves;
+                    ln.kind = even ? LK_DENSE_MFMA2 : LK_DENSE_MFMA;
                     ln.hsp = hsp;
                     ln.P = (maxl + 15) / 16;   // tiles per op
                     const int waves = ln.count * ln.P;
-                    ln.gx = std::max(1, std::min((waves + kMfmaWaves - 1) / kMfmaWaves, c->ncu));
+                    ln.gx = std::max(1, std::min((waves + wpg - 1) / wpg, c->ncu));
                 } else {
                     ln.kind = LK_DENSE;
                     ln.gx = (maxl + kDenseTM - 1) / kDenseTM;
@@ -897,6 +901,16 @@ void launch_one(scft_ctx *c, const Launch &ln)
         else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 32>), g1, b, 0, st, ops, nt, ln.P);
         else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 48>), g1, b, 0, st, ops, nt, ln.P);
         else hipLaunchKernelGGL((k_zdense_mfma<Tin, Tout, 64>), g1, b, 0, st, ops, nt, ln.P);
+        break;
+    }
+    case LK_DENSE_MFMA2: {
+        const DenseOp *ops = c->denseops + ln.first;
+        const dim3 g1(ln.gx), b(64 * kMfma2Waves);
+        const int nt = ln.count * ln.P;
+        if (ln.hsp == 16) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 16>), g1, b, 0, st, ops, nt, ln.P);
+        else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 32>), g1, b, 0, st, ops, nt, ln.P);
+        else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 48>), g1, b, 0, st, ops, nt, ln.P);
+        else hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 64>), g1, b, 0, st, ops, nt, ln.P);
         break;
     }
     case LK_DENSE:
@@ -1154,6 +1168,7 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     c->fft_wave = !(xf && std::strcmp(xf, "lds") == 0);
     const char *dm = std::getenv("SCFT_DENSE");
     c->dense_mfma = !(dm && std::strcmp(dm, "valu") == 0);
+    c->dense_mfma1 = dm && std::strcmp(dm, "mfma1") == 0;
     const char *tw = std::getenv("SCFT_TMP");
     c->tmp_wide = tw && std::strcmp(tw, "fp64") == 0;
     {
@@ -1486,7 +1501,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     double B = 0.0, Fl = 0.0;
     const double ts = double(c->tsize);
     for (const Launch &ln : c->launches) {
-        const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA) ||
+        const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
                           ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE) && ln.pos == 0);
         if ((kind == 0) != is_z) continue;
         if (kind == 1 && ln.pos != 2 && ln.pos != 3) continue;
@@ -1499,7 +1514,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
         auto e_bytes = [](const double *e1, const double *e2, const double *ep) {
             return (e1 ? (e2 && e2 != e1 ? 16.0 : 8.0) : 0.0) + (ep ? 8.0 : 0.0);
         };
-        if (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA) {
+        if (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) {
             std::vector<DenseOp> ops(ln.count);
             HIPCHK(hipMemcpy(ops.data(), c->denseops + ln.first, ln.count * sizeof(DenseOp), hipMemcpyDeviceToHost));
             for (auto &op : ops) {
@@ -1548,11 +1563,12 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
 int scft_get_schedule(const scft_ctx *c, char *buf, int len)
 {
     if (!c || !buf || len <= 0) return SCFT_EINVAL;
-    int n[4] = {0, 0, 0, 0};
-    for (auto &ln : c->launches) n[ln.kind]++;
-    std::snprintf(buf, len, "z=%s launches/iter: lineop=%d dense=%d fftwave=%d dense_mfma=%d graph=%d",
+    int n[5] = {0, 0, 0, 0, 0};
+    for (auto &ln : c->launches)
+        if (ln.kind >= 0 && ln.kind < 5) n[ln.kind]++;
+    std::snprintf(buf, len, "z=%s launches/iter: lineop=%d dense=%d fftwave=%d dense_mfma=%d dense_mfma2=%d graph=%d",
                   c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_DENSE_MFMA],
-                  c->use_graph ? 1 : 0);
+                  n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0);
     return SCFT_OK;
 }
 

@@ -181,10 +181,14 @@ def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, n
         del os.environ["SCFT_XFFT"]
 
 
-@pytest.mark.parametrize("nz,mask", [(128, 1), (127, 1), (48, 1), (64, 0), (20, 1)])
-def test_dense_valu_variant_matches_oracle(abi, configs, orc, lib, nz, mask):
+@pytest.mark.parametrize("variant,nz,mask", [("valu", 128, 1), ("valu", 127, 1), ("valu", 48, 1), ("valu", 64, 0),
+                                              ("valu", 20, 1), ("mfma1", 128, 1), ("mfma1", 96, 0),
+                                              ("mfma1", 20, 1)])
+def test_dense_variant_matches_oracle(abi, configs, orc, lib, variant, nz, mask):
+    """The VALU z operator and the one-wave-per-SIMD MFMA kernel (the default
+    for odd NZ, forced here for even NZ too) against the oracle."""
     os.environ["SCFT_ZKIND"] = "dense"
-    os.environ["SCFT_DENSE"] = "valu"
+    os.environ["SCFT_DENSE"] = variant
     try:
         p = configs.chem_droplet(lx=0.5, ly=1.0, lz=nz * 0.125, dr=0.125, ns=21, vol_dbk=0.2, vol_s1=0.05,
                                  len_s1=0.15)
