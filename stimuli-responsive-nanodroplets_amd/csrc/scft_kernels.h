@@ -33,39 +33,49 @@ constexpr int kDenseHMax = 64;  // k_zdense handles NZ <= 128
 // Global-memory accessors.  Pointers reach the kernels inside argument
 // blocks, so the compiler cannot tell their address space and would emit
 // flat accesses, which also count against the LDS/scalar wait counter and
-// serialise every load with the LDS traffic around it.
+// serialise every load with the LDS traffic around it.  Offsets are 32-bit
+// element offsets from a per-slice (or per-table) base: the base stays in
+// SGPRs and the byte offset in one VGPR (saddr + voffset addressing), with
+// no 64-bit address arithmetic per access.  Slices are limited to 2^28
+// points, so offset * sizeof(T) < 2^32.
 template <typename T>
-__device__ __forceinline__ T ldg(const T *p, size_t i)
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gaddr(const T *p, unsigned i)
 {
-    return ((const __attribute__((address_space(1))) T *)p)[i];
+    return (const __attribute__((address_space(1))) T *)((const char *)p + i * (unsigned)sizeof(T));
 }
 
 template <typename T>
-__device__ __forceinline__ void stg(T *p, size_t i, T v)
+__device__ __forceinline__ T ldg(const T *p, unsigned i)
 {
-    ((__attribute__((address_space(1))) T *)p)[i] = v;
+    return *gaddr(p, i);
 }
 
-// p[i], p[i + 1] in one store (i even, p aligned to two elements).
 template <typename T>
-__device__ __forceinline__ void stg_pair(T *p, size_t i, T a, T b)
+__device__ __forceinline__ void stg(T *p, unsigned i, T v)
 {
-    typedef T v2 __attribute__((ext_vector_type(2)));
-    ((__attribute__((address_space(1))) v2 *)p)[i / 2] = v2{a, b};
+    *(__attribute__((address_space(1))) T *)((char *)p + i * (unsigned)sizeof(T)) = v;
 }
 
-// p[i], p[i + 1] in one load (i even, p aligned to two elements).
 template <typename T>
 struct Pair {
     T x, y;
 };
 
+// p[i], p[i + 1] in one load (i even, p aligned to two elements).
 template <typename T>
-__device__ __forceinline__ Pair<T> ldg_pair(const T *p, size_t i)
+__device__ __forceinline__ Pair<T> ldg_pair(const T *p, unsigned i)
 {
     typedef T v2 __attribute__((ext_vector_type(2)));
-    const v2 v = ((const __attribute__((address_space(1))) v2 *)p)[i / 2];
+    const v2 v = *(const __attribute__((address_space(1))) v2 *)((const char *)p + i * (unsigned)sizeof(T));
     return Pair<T>{v.x, v.y};
+}
+
+// p[i], p[i + 1] in one store (i even, p aligned to two elements).
+template <typename T>
+__device__ __forceinline__ void stg_pair(T *p, unsigned i, T a, T b)
+{
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    *(__attribute__((address_space(1))) v2 *)((char *)p + i * (unsigned)sizeof(T)) = v2{a, b};
 }
 
 // ------------------------------------------------------------ FFT line op
@@ -82,6 +92,7 @@ struct LineOp {
     int nlines;          // real lines
     int c1, s0, s1;      // line base = (l / c1) * s1 + (l % c1) * s0
     int stride;          // element stride along the line
+    int pairs;           // lines 2i, 2i+1 are adjacent and 2-element aligned (nlines even)
     int npass;
     int radix[kMaxPass];
 };
@@ -95,11 +106,11 @@ __device__ __forceinline__ int line_base(const LineOp &op, int l)
 // stored without their post-step Boltzmann factor, so the first axis of the
 // next step applies E_f(t) E_f(t-1): one load squared inside a block
 // (e2 == e1), two loads at a block boundary, E alone on the start slice.
-__device__ __forceinline__ double pre_factor(const double *e1, const double *e2, size_t a)
+__device__ __forceinline__ double pre_factor(const double *e1, const double *e2, unsigned a)
 {
-    const double x = e1[a];
+    const double x = ldg(e1, a);
     if (!e2) return x;
-    return (e2 == e1) ? x * x : x * e2[a];
+    return (e2 == e1) ? x * x : x * ldg(e2, a);
 }
 
 // grid: (line groups, ops); dynamic LDS: 2 * P * Lmax * 16 bytes
@@ -346,10 +357,10 @@ struct TileRegs {
 // for positions past its end) so that they issue back to back; staging
 // zeroes those positions.
 template <int NC>
-__device__ __forceinline__ size_t mfma_addr(int u, int lane, int rbase, int nr, int n)
+__device__ __forceinline__ unsigned mfma_addr(int u, int lane, int rbase, int nr, int n)
 {
     const int r = u / NC, c = lane + 64 * (u % NC);
-    return (size_t)(rbase + (r < nr ? r : 0)) * n + (c < n ? c : 0);
+    return unsigned(rbase + (r < nr ? r : 0)) * unsigned(n) + unsigned(c < n ? c : 0);
 }
 
 // Tiles past the launch's end reload the last tile: the pipeline stays
@@ -420,8 +431,8 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *
     const int tid = threadIdx.x;
     for (int e = tid; e < HSP * HSP; e += 64 * kMfmaWaves) {
         const int j = e / HSP, k = e - j * HSP;
-        Ms[j * LDM + k] = (j < hs && k < hs) ? ldg(ops[0].MsT, (size_t)j * ops[0].ldm + k) : 0.0;
-        Ma[j * LDM + k] = (j < ha && k < ha) ? ldg(ops[0].MaT, (size_t)j * ops[0].ldm + k) : 0.0;
+        Ms[j * LDM + k] = (j < hs && k < hs) ? ldg(ops[0].MsT, unsigned(j * ops[0].ldm + k)) : 0.0;
+        Ma[j * LDM + k] = (j < ha && k < ha) ? ldg(ops[0].MaT, unsigned(j * ops[0].ldm + k)) : 0.0;
     }
     __syncthreads();
     const int lane = tid & 63;
@@ -485,7 +496,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *
             for (int r = 0; r < 4; ++r) {
                 const int rr = lj + 4 * r;
                 if (rr >= nr) continue;
-                const size_t row = (size_t)(rbase + rr) * n;
+                const unsigned row = unsigned(rbase + rr) * unsigned(n);
                 const double u = U[kt][r], v = W[kt][r];
                 double y = u + v;
                 if (op.epost) y *= ldg(op.epost, row + k);
@@ -527,8 +538,8 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
     for (int e = tid; e < HSP * HSP; e += 64 * kMfma2Waves) {
         const int j = e / HSP, k = e - j * HSP;
         const bool in = j < hs && k < hs;
-        Ms[j * LDM + k] = in ? ldg(ops[0].MsT, (size_t)j * ops[0].ldm + k) : 0.0;
-        Ma[j * LDM + k] = in ? ldg(ops[0].MaT, (size_t)j * ops[0].ldm + k) : 0.0;
+        Ms[j * LDM + k] = in ? ldg(ops[0].MsT, unsigned(j * ops[0].ldm + k)) : 0.0;
+        Ma[j * LDM + k] = in ? ldg(ops[0].MaT, unsigned(j * ops[0].ldm + k)) : 0.0;
     }
     __syncthreads();
     const int lane = tid & 63;
@@ -547,9 +558,10 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
         const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? 2 : 3);
         Pair<Tin> xa[16];
         Pair<double> ea[16];
+        const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(cc);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const size_t a = (size_t)(rbase + (r < nr ? r : 0)) * n + cc;
+            const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
             xa[r] = ldg_pair(src, a);
             ea[r] = ldg_pair(op.epre, a);
         }
@@ -566,7 +578,7 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
                         e0 *= e0;
                         e1 *= e1;
                     } else if (mode == 3) {   // one step per block: load the second factor here
-                        const Pair<double> f = ldg_pair(op.epre2, (size_t)(rbase + (R < nr ? R : 0)) * n + cc);
+                        const Pair<double> f = ldg_pair(op.epre2, o0 + unsigned(R < nr ? R : 0) * unsigned(n));
                         e0 *= f.x;
                         e1 *= f.y;
                     }
@@ -615,7 +627,7 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
             for (int r = 0; r < 4; ++r) {
                 const int rr = lj + 4 * r;
                 if (rr >= nr) continue;
-                const size_t row = (size_t)(rbase + rr) * n;
+                const unsigned row = unsigned(rbase + rr) * unsigned(n);
                 const double u = U[kt][r], v = W[kt][r];
                 stg(dst, row + k, Tout(u + v));
                 stg(dst, row + n - 1 - k, Tout(u - v));
@@ -842,6 +854,179 @@ __global__ void __launch_bounds__(64 * kFftWaves) k_fftwave(const LineOp *__rest
         double v = reinterpret_cast<const double *>(&buf[(r >> 1) * LS + t])[r & 1] * scale;
         if (op.epost) v *= ldg(op.epost, a);
         stg(dst, a, Tout(v));
+    }
+}
+
+
+// ------------------------------------- periodic FFT of length 256, in registers
+//
+// Four-step transform, 256 = 16 x 16: lane p of a 16-lane group holds
+// x[p + 16 m] (m = 0..15) of one complex line (two real grid lines), does a
+// 16-point DFT over m in registers, multiplies by w_256^{p k2}, and one LDS
+// transpose gives lane q the column k2 = q, whose 16-point DFT over p yields
+// X[q + 16 k1].  K is applied in that order and the inverse runs the same
+// steps backwards, so the spectrum is never reordered.  Each line makes two
+// LDS passes (one transpose per direction) instead of one per radix pass.
+// A workgroup of 4 waves takes 16 complex lines = 32 consecutive real lines:
+// for strided (x, y) lines one grid row of those 32 lines is one 128-byte
+// segment, loaded as point pairs of two adjacent lines.
+
+constexpr int kF16Waves = 4;
+constexpr int kF16Lines = 2 * 4 * kF16Waves;   // real lines per workgroup
+
+// w_16^{m} = exp(SIGN 2 pi i m / 16)
+template <int SIGN>
+__device__ __forceinline__ cpx<double> w16(int m)
+{
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, r2 = 0.70710678118654752440;
+    const double C[16] = {1.0, c1, r2, s1, 0.0, -s1, -r2, -c1, -1.0, -c1, -r2, -s1, 0.0, s1, r2, c1};
+    const double S[16] = {0.0, s1, r2, c1, 1.0, c1, r2, s1, 0.0, -s1, -r2, -c1, -1.0, -c1, -r2, -s1};
+    return cpx<double>{C[m & 15], SIGN * S[m & 15]};
+}
+
+// In-place 16-point DFT with w = exp(SIGN 2 pi i / 16), natural order in and out.
+template <int SIGN>
+__device__ __forceinline__ void dft16(cpx<double> (&v)[16])
+{
+    cpx<double> t[4][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        cpx<double> a[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+        dft_small<4>(a, double(SIGN));
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) t[n1][k2] = (n1 * k2 == 0) ? a[k2] : cmul(a[k2], w16<SIGN>(n1 * k2));
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+        cpx<double> a[4] = {t[0][k2], t[1][k2], t[2][k2], t[3][k2]};
+        dft_small<4>(a, double(SIGN));
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = a[k1];
+    }
+}
+
+
+
+// Loads of one unit (4 complex lines of a wave) held until the unit starts.
+template <typename Tin>
+struct F16Regs {
+    Pair<Tin> x[16];
+};
+
+// Unit u of the launch: op u / upo, real lines 8 (u % upo) + 2 g, +1.
+template <typename Tin>
+__device__ __forceinline__ void f16_fetch(const LineOp *__restrict__ ops, int upo, int u, int nunits, int g, int p,
+                                          F16Regs<Tin> &R)
+{
+    u = min(u, nunits - 1);
+    const LineOp &op = ops[u / upo];
+    const int r0 = (u % upo) * 8 + 2 * g;
+    const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
+    const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
+    const Tin *src = static_cast<const Tin *>(op.src);
+    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = 16u * unsigned(op.stride);
+    if (op.pairs) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) R.x[m] = ldg_pair(src, o0 + m * step);
+    } else {
+        const unsigned o1 = o0 + unsigned(b1 - b0);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) R.x[m] = Pair<Tin>{ldg(src, o0 + m * step), ldg(src, o1 + m * step)};
+    }
+}
+
+// Persistent: each wave strides over units; the next unit's loads are in
+// flight during the current unit's transforms.  The 256 twiddles sit in LDS
+// once per workgroup (every op of a launch has the same length).
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(64 * kF16Waves) k_fft256(const LineOp *__restrict__ ops, int nunits, int upo)
+{
+    constexpr int L = 256, RS = 17;   // transpose row stride (complex): odd, so columns spread over banks
+    __shared__ cpx<double> T[kF16Waves * 4 * 16 * RS];
+    __shared__ cpx<double> TW[L];
+    const int tid = threadIdx.x, lane = tid & 63;
+    {
+        const double *twd = reinterpret_cast<const double *>(ops[0].tw);
+        for (int e = tid; e < L; e += 64 * kF16Waves) TW[e] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
+    }
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // line-minor lanes: the four lanes of one grid row are adjacent, so a
+    // quarter-wave touches 4 cache lines per load instead of 16
+    const int g = lane & 3, p = lane >> 2;
+    cpx<double> *Tg = T + (wave * 4 + g) * 16 * RS;
+    const int ustride = gridDim.x * kF16Waves;
+    int u = blockIdx.x * kF16Waves + wave;
+    if (u >= nunits) return;   // no workgroup barrier follows
+    F16Regs<Tin> R;
+    f16_fetch<Tin>(ops, upo, u, nunits, g, p, R);
+    for (; u < nunits; u += ustride) {
+        const LineOp &op = ops[u / upo];
+        const int r0 = (u % upo) * 8 + 2 * g;
+        const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
+        const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
+        const int stride = op.stride;
+        cpx<double> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = cpx<double>{double(R.x[m].x), double(R.x[m].y)};
+        f16_fetch<Tin>(ops, upo, u + ustride, nunits, g, p, R);   // in flight during the transforms
+        const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(stride), step = 16u * unsigned(stride);
+        const unsigned o1 = o0 + unsigned(b1 - b0);
+        if (op.epre) {   // first axis of a step (periodic z): not prefetched
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                v[m].x *= pre_factor(op.epre, op.epre2, o0 + m * step);
+                v[m].y *= pre_factor(op.epre, op.epre2, o1 + m * step);
+            }
+        }
+        // forward: over m, twiddle w^{-p k2}, transpose, over p
+        dft16<-1>(v);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], TW[(p * k) & (L - 1)]);
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) Tg[k * RS + p] = v[k];
+        wave_sync();
+        const int q = p;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = Tg[q * RS + k];
+        dft16<-1>(v);   // v[k1] = X[q + 16 k1]
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const double kk = ldg(op.K, q + 16 * k1) * (1.0 / L);   // 1/L folded in (exact)
+            v[k1].x *= kk;
+            v[k1].y *= kk;
+        }
+        // inverse: over k1 (index p out), twiddle w^{+p q}, transpose, over q
+        dft16<+1>(v);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) {
+            cpx<double> w = TW[(q * k) & (L - 1)];
+            w.y = -w.y;
+            v[k] = cmul(v[k], w);
+        }
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) Tg[q * RS + k] = v[k];
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = Tg[k * RS + p];
+        dft16<+1>(v);   // v[m] = x[p + 16 m] after the operator
+        Tout *dst = static_cast<Tout *>(op.dst);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            double y0 = v[m].x, y1 = v[m].y;
+            if (op.epost) {
+                y0 *= ldg(op.epost, o0 + m * step);
+                y1 *= ldg(op.epost, o1 + m * step);
+            }
+            if (op.pairs) {
+                if (ok0) stg_pair(dst, o0 + m * step, Tout(y0), Tout(y1));
+            } else {
+                if (ok0) stg(dst, o0 + m * step, Tout(y0));
+                if (ok1) stg(dst, o1 + m * step, Tout(y1));
+            }
+        }
     }
 }
 

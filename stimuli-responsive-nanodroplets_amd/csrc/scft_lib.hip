@@ -199,7 +199,8 @@ struct Slot {
     std::vector<QSlot> qslots;
 };
 
-enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4 };
+enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFT256 = 5 };
+constexpr int kLaunchKinds = 6;
 
 struct Launch {
     int kind;        // LaunchKind
@@ -233,6 +234,7 @@ struct scft_ctx {
     std::vector<std::unique_ptr<DenseTable>> denses;
     int zkind = Z_FFT;
     bool fft_wave = true;            // wave-per-line FFT for supported periodic lengths
+    bool fft_reg = true;             // four-step register FFT for periodic length 256
     bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
     bool dense_mfma1 = false;        // force the one-wave-per-SIMD MFMA kernel for even NZ too
     bool tmp_wide = false;           // FP64 scratch between axis operators in FP32 mode
@@ -737,6 +739,8 @@ int build_schedule(scft_ctx *c)
                         op.nlines = s.NY * s.NZ;
                         op.c1 = s.NY * s.NZ; op.s0 = 1; op.s1 = 0; op.stride = s.NY * s.NZ;
                     }
+                    // strided lines 2i, 2i+1 sit next to each other at an even offset
+                    op.pairs = (op.s0 == 1 && op.c1 % 2 == 0 && op.s1 % 2 == 0 && op.stride % 2 == 0) ? 1 : 0;
                     tl.push_back(op);
                 }
             }
@@ -785,7 +789,13 @@ int build_schedule(scft_ctx *c)
                     }
                 ln.count = int(lops.size()) - ln.first;
                 ln.L = L;
-                if (c->fft_wave && !dirichlet && fftwave_supported(L)) {
+                if (c->fft_reg && !dirichlet && L == 256) {
+                    // persistent waves over (op, 8-line unit); two workgroups per CU
+                    ln.kind = LK_FFT256;
+                    ln.P = (maxl + 7) / 8;   // units per op
+                    const int units = ln.count * ln.P;
+                    ln.gx = std::max(1, std::min((units + kF16Waves - 1) / kF16Waves, 2 * c->ncu));
+                } else if (c->fft_wave && !dirichlet && fftwave_supported(L)) {
                     ln.kind = LK_FFTWAVE;
                     ln.gx = (maxl + 2 * kFftWaves - 1) / (2 * kFftWaves);
                 } else {
@@ -918,6 +928,10 @@ void launch_one(scft_ctx *c, const Launch &ln)
         break;
     case LK_FFTWAVE:
         launch_fftwave<Tin, Tout>(c, ln);
+        break;
+    case LK_FFT256:
+        hipLaunchKernelGGL((k_fft256<Tin, Tout>), dim3(ln.gx), dim3(64 * kF16Waves), 0, st, c->lineops + ln.first,
+                           ln.count * ln.P, ln.P);
         break;
     default:
         hipLaunchKernelGGL((k_lineop<Tin, Tout>), g, dim3(kThreads), ln.lds, st, c->lineops + ln.first, ln.P);
@@ -1151,7 +1165,7 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     c->NZ = c->dproto.nz;
     c->max_nx = std::max(max_nx, c->dproto.nx);
     const size_t V = size_t(c->max_nx) * c->NY * c->NZ;
-    if (V > size_t(1) << 30) {
+    if (V > size_t(1) << 28) {   // kernels address a slice with 32-bit byte offsets
         set_err("grid too large");
         delete c;
         return SCFT_EUNSUPPORTED;
@@ -1166,6 +1180,7 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     // SCFT_TMP=fp64 FP64 scratch between axis operators in FP32 mode
     const char *xf = std::getenv("SCFT_XFFT");
     c->fft_wave = !(xf && std::strcmp(xf, "lds") == 0);
+    c->fft_reg = !(xf && (std::strcmp(xf, "lds") == 0 || std::strcmp(xf, "wave") == 0));
     const char *dm = std::getenv("SCFT_DENSE");
     c->dense_mfma = !(dm && std::strcmp(dm, "valu") == 0);
     c->dense_mfma1 = dm && std::strcmp(dm, "mfma1") == 0;
@@ -1502,7 +1517,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     const double ts = double(c->tsize);
     for (const Launch &ln : c->launches) {
         const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
-                          ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE) && ln.pos == 0);
+                          ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFT256) && ln.pos == 0);
         if ((kind == 0) != is_z) continue;
         if (kind == 1 && ln.pos != 2 && ln.pos != 3) continue;
         sel.push_back(ln);
@@ -1563,12 +1578,13 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
 int scft_get_schedule(const scft_ctx *c, char *buf, int len)
 {
     if (!c || !buf || len <= 0) return SCFT_EINVAL;
-    int n[5] = {0, 0, 0, 0, 0};
+    int n[kLaunchKinds] = {};
     for (auto &ln : c->launches)
-        if (ln.kind >= 0 && ln.kind < 5) n[ln.kind]++;
-    std::snprintf(buf, len, "z=%s launches/iter: lineop=%d dense=%d fftwave=%d dense_mfma=%d dense_mfma2=%d graph=%d",
-                  c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_DENSE_MFMA],
-                  n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0);
+        if (ln.kind >= 0 && ln.kind < kLaunchKinds) n[ln.kind]++;
+    std::snprintf(buf, len,
+                  "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fft256=%d dense_mfma=%d dense_mfma2=%d graph=%d",
+                  c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFT256],
+                  n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0);
     return SCFT_OK;
 }
 

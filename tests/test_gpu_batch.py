@@ -150,18 +150,21 @@ def test_z_operator_sizes_match_oracle(abi, configs, orc, lib, zkind, nz, mask):
         del os.environ["SCFT_ZKIND"]
 
 
-@pytest.mark.parametrize("variant", ["wave", "lds"])
-@pytest.mark.parametrize("nx,ny", [(16, 1), (48, 1), (80, 1), (96, 1), (160, 1), (192, 1), (320, 1), (384, 1),
-                                   (512, 1), (12, 1), (32, 16), (16, 48), (64, 64)])
-def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, ny):
-    """x and y axes through the wave-per-line FFT (compile-time lengths) and
-    the generic LDS FFT, against the oracle, on a thin NZ = 6 film."""
+@pytest.mark.parametrize("variant", ["reg", "wave", "lds"])
+@pytest.mark.parametrize("nx,ny,nz", [(16, 1, 6), (48, 1, 6), (80, 1, 6), (96, 1, 6), (160, 1, 6), (192, 1, 6),
+                                      (320, 1, 6), (384, 1, 6), (512, 1, 6), (12, 1, 6), (32, 16, 6), (16, 48, 6),
+                                      (64, 64, 6), (256, 1, 6), (256, 8, 6), (32, 256, 6), (256, 3, 5),
+                                      (8, 256, 5)])
+def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, ny, nz):
+    """x and y axes through the four-step register FFT (length 256), the
+    wave-per-line FFT (compile-time lengths) and the generic LDS FFT, against
+    the oracle, on a thin film; odd NZ makes the strided lines odd-aligned."""
     os.environ["SCFT_XFFT"] = variant
     try:
-        p = configs.chem_droplet(lx=nx * 0.25, ly=(ny * 0.25 if ny > 1 else 1.0), lz=1.5, dr=0.25, ns=11,
+        p = configs.chem_droplet(lx=nx * 0.25, ly=(ny * 0.25 if ny > 1 else 1.0), lz=nz * 0.25, dr=0.25, ns=11,
                                  vol_dbk=0.2, vol_s1=0.05, len_s1=0.3)
         ctx = abi.Context(lib, p, abi.SCFT_FP64)
-        assert ctx.shape == (nx, ny, 6)
+        assert ctx.shape == (nx, ny, nz)
         sched = ctx.schedule()
         ref = orc.state(p)
         d = ctx.derived
