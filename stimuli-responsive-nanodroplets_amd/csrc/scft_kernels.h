@@ -1166,12 +1166,14 @@ __device__ __forceinline__ double simpson(const DenTerm &t, int V, int idx)
     const int top = t.ns - 1;
     const T *q = static_cast<const T *>(t.q);
     const T *qd = static_cast<const T *>(t.qd);
-    const double qa = t.eqa[idx], qb = t.eqb[idx], da = t.eda[idx], db = t.edb[idx];
+    const unsigned ui = unsigned(idx);
+    const double qa = ldg(t.eqa, ui), qb = ldg(t.eqb, ui), da = ldg(t.eda, ui), db = ldg(t.edb, ui);
     auto fac = [](int s, int sw, double a, double b) { return s == 0 ? 1.0 : (s <= sw ? a : b); };
+    // slice base pointers are uniform (scalar arithmetic), the point offset is 32-bit
     auto qc = [&](int s) -> double {
         const int u = top - s;
-        return (double(q[(size_t)s * sV + idx]) * fac(s, t.qsw, qa, qb)) *
-               (double(qd[(size_t)u * sV + idx]) * fac(u, t.dsw, da, db));
+        return (double(ldg(q + (size_t)s * sV, ui)) * fac(s, t.qsw, qa, qb)) *
+               (double(ldg(qd + (size_t)u * sV, ui)) * fac(u, t.dsw, da, db));
     };
     if constexpr (sizeof(T) == 8) {
         // the reference's order: ends, then 4*odd, then 2*even (SCFT.c:2283-2297)
@@ -1182,11 +1184,21 @@ __device__ __forceinline__ double simpson(const DenTerm &t, int V, int idx)
     } else {
         // FP32 slices: products and the contour sum in FP64 with Kahan compensation
         double sum = qc(t.start) + qc(t.stop), comp = 0.0;
-        for (int s = t.start + 1; s <= t.stop - 1; ++s) {
-            const double y = (((s - t.start) & 1) ? 4.0 : 2.0) * qc(s) - comp;
+        auto add = [&](double y) {
+            y -= comp;
             const double tt = sum + y;
             comp = (tt - sum) - y;
             sum = tt;
+        };
+        if (t.q == t.qd && t.start == 0 && t.stop == top && !(top & 1)) {
+            // one chain over the whole contour (homopolymer): qc(s) = qc(top - s)
+            // and the weights are symmetric, so each slice is read once
+#pragma unroll 8
+            for (int s = 1; s < top / 2; ++s) add(((s & 1) ? 8.0 : 4.0) * qc(s));
+            if (top > 0) add((((top / 2) & 1) ? 4.0 : 2.0) * qc(top / 2));
+        } else {
+#pragma unroll 8
+            for (int s = t.start + 1; s <= t.stop - 1; ++s) add((((s - t.start) & 1) ? 4.0 : 2.0) * qc(s));
         }
         return sum;
     }
