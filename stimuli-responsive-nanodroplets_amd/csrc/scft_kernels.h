@@ -941,20 +941,26 @@ __device__ __forceinline__ void f16_fetch(const LineOp *__restrict__ ops, int up
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(64 * kF16Waves) k_fft256(const LineOp *__restrict__ ops, int nunits, int upo)
 {
-    constexpr int L = 256, RS = 17;   // transpose row stride (complex): odd, so columns spread over banks
-    __shared__ cpx<double> T[kF16Waves * 4 * 16 * RS];
-    __shared__ cpx<double> TW[L];
+    // LDS banks: a b128 access is served 16 lanes at a time, i.e. 4 lines (g)
+    // x 4 points (p).  Rows of a line's 16x16 tile are RS = 17 complex apart
+    // (columns spread over the banks) and line tiles RG = 16 RS + 4 apart,
+    // so the 4 lines of a phase start 16 banks apart.  The twiddle table has
+    // one pad entry per 16.
+    constexpr int L = 256, RS = 17, RG = 16 * RS + 4;
+    __shared__ cpx<double> T[kF16Waves * 4 * RG];
+    __shared__ cpx<double> TW[L + L / 16];
     const int tid = threadIdx.x, lane = tid & 63;
     {
         const double *twd = reinterpret_cast<const double *>(ops[0].tw);
-        for (int e = tid; e < L; e += 64 * kF16Waves) TW[e] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
+        for (int e = tid; e < L; e += 64 * kF16Waves)
+            TW[e + (e >> 4)] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
     }
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // line-minor lanes: the four lanes of one grid row are adjacent, so a
     // quarter-wave touches 4 cache lines per load instead of 16
     const int g = lane & 3, p = lane >> 2;
-    cpx<double> *Tg = T + (wave * 4 + g) * 16 * RS;
+    cpx<double> *Tg = T + (wave * 4 + g) * RG;
     const int ustride = gridDim.x * kF16Waves;
     int u = blockIdx.x * kF16Waves + wave;
     if (u >= nunits) return;   // no workgroup barrier follows
@@ -982,7 +988,10 @@ __global__ void __launch_bounds__(64 * kF16Waves) k_fft256(const LineOp *__restr
         // forward: over m, twiddle w^{-p k2}, transpose, over p
         dft16<-1>(v);
 #pragma unroll
-        for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], TW[(p * k) & (L - 1)]);
+        for (int k = 1; k < 16; ++k) {
+            const int e = (p * k) & (L - 1);
+            v[k] = cmul(v[k], TW[e + (e >> 4)]);
+        }
         wave_sync();
 #pragma unroll
         for (int k = 0; k < 16; ++k) Tg[k * RS + p] = v[k];
@@ -1001,7 +1010,8 @@ __global__ void __launch_bounds__(64 * kF16Waves) k_fft256(const LineOp *__restr
         dft16<+1>(v);
 #pragma unroll
         for (int k = 1; k < 16; ++k) {
-            cpx<double> w = TW[(q * k) & (L - 1)];
+            const int e = (q * k) & (L - 1);
+            cpx<double> w = TW[e + (e >> 4)];
             w.y = -w.y;
             v[k] = cmul(v[k], w);
         }
