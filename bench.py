@@ -35,6 +35,22 @@ for _p in (ROOT, os.path.join(ROOT, "oracle"), PKG):
 
 PEAK_HBM_GBS = 8000.0
 CONV_PROFILE = os.path.join(ROOT, "profiles", "c2_convergence_%s.json")   # per precision
+# HBM bytes per z-operator dispatch from the FETCH_SIZE / WRITE_SIZE runs
+# (calibrated with bin/copy_probe); see profiles/c2_served_b192_hbm_traffic.md
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c2_served_b192_hbm_traffic.json")
+
+
+def measured_traffic(schedule, slots, precision):
+    """Counter-measured bytes per launch of the dominant kernel, when the
+    committed measurement is of the same kernel and workload (192 slots,
+    FP32-split); None otherwise."""
+    if not os.path.exists(TRAFFIC_PROFILE) or slots != 192 or precision != "fp32":
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        t = json.load(f)
+    if t.get("kernel") != "k_zdense_mfma2" or "dense_mfma2=100" not in schedule:
+        return None
+    return float(t["bytes_per_launch"])
 
 
 def log(*a):
@@ -178,7 +194,8 @@ def main():
     ms_per_step = 1000.0 * dt / args.steps
     inst_iter_per_s = inst_iters / dt
     log("timed: %d steps x %d slots in %.3f s (%.3f ms/step)" % (args.steps, B, dt, ms_per_step))
-    log(pool.batch.schedule())
+    sched = pool.batch.schedule()
+    log(sched)
 
     # dominant kernel (the z operator) timed with HIP events on the pool's stream
     zt = pool.batch.time_launches(kind=0, reps=5)
@@ -236,8 +253,9 @@ def main():
                                          "solves_per_hour": single_sph},
                    "speedup_vs_fp64_single": (value / single_sph) if (value and single_sph) else None},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": (ach / PEAK_HBM_GBS) if ach else None, "traffic": None,
-                     "kernel": "k_zdense_mfma (z operator of every chain, one launch per contour step)",
+                     "frac": (ach / PEAK_HBM_GBS) if ach else None,
+                     "traffic": measured_traffic(sched, B, args.precision),
+                     "kernel": "k_zdense_mfma2 (z operator of every chain, one launch per contour step)",
                      "launch_us": zt["avg_us"], "algorithmic_bytes_per_launch": zt["bytes"],
                      "fp64_tflops": zt["flops"] / (zt["avg_us"] * 1e-6) / 1e12 if zt["avg_us"] > 0 else None,
                      "x_axis_kernel": {"launch_us": xt["avg_us"], "bytes": xt["bytes"],
