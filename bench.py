@@ -113,17 +113,32 @@ def cpu_baseline(abi, configs, seconds):
     return (time.perf_counter() - t0) / n, n
 
 
-def converge_run(abi, configs, serve, lib, nsolves, prec_name, tlimit, out, fields_dir=None):
+def converge_run(abi, configs, serve, lib, nsolves, prec_name, tlimit, out, fields_dir=None, state_in=None,
+                 state_out=None):
     """Run nsolves C2 solves (droplet seeds 0..nsolves-1, all resident) to
     convergence; write their iterations-to-converge, converged F and
-    ||dw|| history to `out` (JSON)."""
+    ||dw|| history to `out` (JSON).  With `state_in`, the solves continue
+    from saved states (fields, iteration count, history) instead; with
+    `state_out`, the solves still running at the time limit are saved there
+    (the iteration is deterministic, so a resumed solve continues exactly)."""
     import numpy as np
     precision = abi.SCFT_FP32_SPLIT if prec_name == "fp32" else abi.SCFT_FP64
     p = configs.params_c2()
     pool = serve.Pool(lib, p, precision=precision, max_slots=nsolves, record_every=1000)
-    for i in range(nsolves):
-        pool.submit(serve.Request(i, configs.params_c2(), (serve.SEED_DROPLET, {"radius": 6.0, "seed": i}),
-                                  keep_fields=fields_dir is not None))
+    if state_in:
+        for name in sorted(os.listdir(state_in)):
+            if not name.endswith(".npz"):
+                continue
+            z = np.load(os.path.join(state_in, name))
+            r = serve.Request(int(z["seed"]), configs.params_c2(), (serve.SEED_FIELDS, (z["w"], z["p"])),
+                              keep_fields=fields_dir is not None)
+            r.iters = int(z["iters"])
+            r.history = [tuple(h) for h in z["history"].tolist()]
+            pool.submit(r)
+    else:
+        for i in range(nsolves):
+            pool.submit(serve.Request(i, configs.params_c2(), (serve.SEED_DROPLET, {"radius": 6.0, "seed": i}),
+                                      keep_fields=fields_dir is not None))
     t0 = time.perf_counter()
     last = t0
     while pool.busy() and time.perf_counter() - t0 < tlimit:
@@ -147,9 +162,14 @@ def converge_run(abi, configs, serve, lib, nsolves, prec_name, tlimit, out, fiel
                 w, pf = r.fields
                 np.savez_compressed(os.path.join(fields_dir, "%s_seed%d.npz" % (prec_name, r.rid)), w=w, p=pf)
         res["history"][str(r.rid)] = r.history
-    for r in pool.running.values():
+    for s, r in pool.running.items():
         res["unconverged"].append({"seed": r.rid, "iters": r.iters, "last": r.history[-1] if r.history else None})
         res["history"][str(r.rid)] = r.history
+        if state_out:
+            os.makedirs(state_out, exist_ok=True)
+            w, pf = pool.batch.get_fields(s)
+            np.savez_compressed(os.path.join(state_out, "%s_seed%d.npz" % (prec_name, r.rid)), w=w, p=pf, seed=r.rid,
+                     iters=r.iters, history=np.asarray(r.history, float).reshape(-1, 3))
     pool.close()
     with open(out, "w") as f:
         json.dump(res, f)
@@ -171,6 +191,8 @@ def main():
     ap.add_argument("--converge-seconds", type=float, default=1000.0)
     ap.add_argument("--converge-out", default=None, help="output JSON of --converge (default: the profile)")
     ap.add_argument("--converge-fields", default=None, help="directory for the converged fields (npz)")
+    ap.add_argument("--converge-state-in", default=None, help="continue the solves saved in this directory")
+    ap.add_argument("--converge-state-out", default=None, help="save solves still running at the limit here")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only (one process, one MI355X); --gpus %d is not supported" % args.gpus)
@@ -183,7 +205,8 @@ def main():
 
     if args.converge:
         converge_run(abi, configs, serve, lib, args.converge, args.precision, args.converge_seconds,
-                     args.converge_out or CONV_PROFILE % args.precision, args.converge_fields)
+                     args.converge_out or CONV_PROFILE % args.precision, args.converge_fields,
+                     args.converge_state_in, args.converge_state_out)
         return
 
     precision = abi.SCFT_FP32_SPLIT if args.precision == "fp32" else abi.SCFT_FP64
