@@ -253,7 +253,16 @@ def main():
                          % (n_cpu, t_cpu),
                "s_per_iteration": t_cpu}
 
-    ach = zt["bytes"] / (zt["avg_us"] * 1e-6) / 1e9 if zt["avg_us"] > 0 else None
+    # achieved: the path's algorithmic bytes per unit (a propagator voxel-step
+    # reads q_s and writes q_{s+1}: 2 b bytes, b = 4 in FP32-split, 8 in FP64;
+    # E and K are per-block operands and not counted) times the voxel-steps one
+    # z launch processes, over the z launch's duration.  The kernel itself also
+    # reads E (8 B per voxel-step); its bytes are reported beside.
+    b_el = 4.0 if args.precision == "fp32" else 8.0
+    units = zt["voxel_steps"]
+    alg_bytes = 2.0 * b_el * units
+    ach = alg_bytes / (zt["avg_us"] * 1e-6) / 1e9 if zt["avg_us"] > 0 else None
+    kern_gbs = zt["bytes"] / (zt["avg_us"] * 1e-6) / 1e9 if zt["avg_us"] > 0 else None
     line = {
         "metric": "converged SCFT solves/hour (p50/p99 time-to-converge at ||dw||_rms < 1e-5)",
         "value": value,
@@ -279,7 +288,9 @@ def main():
                      "frac": (ach / PEAK_HBM_GBS) if ach else None,
                      "traffic": measured_traffic(sched, B, args.precision),
                      "kernel": "k_zdense_mfma2 (z operator of every chain, one launch per contour step)",
-                     "launch_us": zt["avg_us"], "algorithmic_bytes_per_launch": zt["bytes"],
+                     "launch_us": zt["avg_us"], "voxel_steps_per_launch": units,
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel_bytes_per_launch": zt["bytes"], "kernel_GBs": kern_gbs,
                      "fp64_tflops": zt["flops"] / (zt["avg_us"] * 1e-6) / 1e12 if zt["avg_us"] > 0 else None,
                      "x_axis_kernel": {"launch_us": xt["avg_us"], "bytes": xt["bytes"],
                                        "GBs": xt["bytes"] / (xt["avg_us"] * 1e-6) / 1e9 if xt["avg_us"] > 0 else None}},

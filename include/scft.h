@@ -180,11 +180,13 @@ int scft_slot_bytes(const scft_params *p, int max_nx, int precision, size_t *byt
 int scft_device_memory(size_t *free_bytes, size_t *total_bytes);
 /* Measurement hook: run every launch of one kernel kind of the current
  * schedule (kind 0: z operator, 1: last axis) reps times between two HIP
- * events on the context's stream; returns the mean launch duration (us)
- * and the algorithmic bytes and flops of the mean launch.  Overwrites the
- * chains' scratch and propagator slices, so call it between solves. */
+ * events on the context's stream; returns the mean launch duration (us),
+ * the bytes the kernel reads and writes (slices, scratch and Boltzmann
+ * factors), the flops and the voxel-steps (grid points times chains) of
+ * the mean launch.  Overwrites the chains' scratch and propagator slices,
+ * so call it between solves.  No counterpart in the reference. */
 int scft_time_launches(scft_ctx *ctx, int kind, int reps, double *avg_us, double *bytes, double *flops,
-                       int *nlaunch);
+                       double *voxel_steps, int *nlaunch);
 /* Human-readable schedule summary (z operator kind, launches per iteration). */
 int scft_get_schedule(const scft_ctx *ctx, char *buf, int len);
 

@@ -167,7 +167,7 @@ def load_library(path=None):
         "scft_get_schedule": ([vp, C.c_char_p, C.c_int], C.c_int),
         "scft_slot_bytes": ([P(ScftParams), C.c_int, C.c_int, P(C.c_size_t)], C.c_int),
         "scft_device_memory": ([P(C.c_size_t), P(C.c_size_t)], C.c_int),
-        "scft_time_launches": ([vp, C.c_int, C.c_int, dp, dp, dp, P(C.c_int)], C.c_int),
+        "scft_time_launches": ([vp, C.c_int, C.c_int, dp, dp, dp, dp, P(C.c_int)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -350,9 +350,10 @@ class Batch:
         return buf.value.decode()
 
     def time_launches(self, kind=0, reps=10):
-        """Mean duration (us), algorithmic bytes and flops of one launch of the
-        z-operator (kind 0) or last-axis (kind 1) kernel of this schedule."""
-        us, b, f, n = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        """Mean duration (us), kernel bytes, flops and voxel-steps of one launch
+        of the z-operator (kind 0) or last-axis (kind 1) kernel of this schedule."""
+        us, b, f, v, n = C.c_double(), C.c_double(), C.c_double(), C.c_double(), C.c_int()
         check(self.lib, self.lib.scft_time_launches(self.h, kind, reps, C.byref(us), C.byref(b), C.byref(f),
-                                                    C.byref(n)), "scft_time_launches")
-        return {"avg_us": us.value, "bytes": b.value, "flops": f.value, "launches": n.value}
+                                                    C.byref(v), C.byref(n)), "scft_time_launches")
+        return {"avg_us": us.value, "bytes": b.value, "flops": f.value, "voxel_steps": v.value,
+                "launches": n.value}

@@ -1508,12 +1508,12 @@ int scft_sync(scft_ctx *c)
 // stream; report the mean launch duration and the algorithmic bytes and
 // flops of the mean launch.  kind: 0 = z operator, 1 = last (x) axis.
 int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *bytes, double *flops,
-                       int *nlaunch)
+                       double *voxel_steps, int *nlaunch)
 {
-    if (!c || reps < 1 || !avg_us || !bytes || !flops || !nlaunch) return SCFT_EINVAL;
+    if (!c || reps < 1 || !avg_us || !bytes || !flops || !voxel_steps || !nlaunch) return SCFT_EINVAL;
     if (c->dirty) RETCHK(build_schedule(c));
     std::vector<Launch> sel;
-    double B = 0.0, Fl = 0.0;
+    double B = 0.0, Fl = 0.0, U = 0.0;
     const double ts = double(c->tsize);
     for (const Launch &ln : c->launches) {
         const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
@@ -1534,6 +1534,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
             HIPCHK(hipMemcpy(ops.data(), c->denseops + ln.first, ln.count * sizeof(DenseOp), hipMemcpyDeviceToHost));
             for (auto &op : ops) {
                 const double elems = double(op.nrows) * op.n;
+                U += elems;
                 B += elems * (in_b + out_b + e_bytes(op.epre, op.epre2, op.epost));
                 Fl += 2.0 * op.nrows * (double(op.hs) * op.hs + double(op.ha) * op.ha);
             }
@@ -1542,6 +1543,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
             HIPCHK(hipMemcpy(ops.data(), c->lineops + ln.first, ln.count * sizeof(LineOp), hipMemcpyDeviceToHost));
             for (auto &op : ops) {
                 const double elems = double(op.nlines) * op.n;
+                U += elems;
                 B += elems * (in_b + out_b + e_bytes(op.epre, op.epre2, op.epost));
                 // radix-2-equivalent count of a complex transform pair: 2 x 5 L log2 L per packed line pair
                 Fl += (op.nlines / 2.0) * 2.0 * 5.0 * op.L * std::log2(double(op.L));
@@ -1550,7 +1552,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     }
     *nlaunch = int(sel.size());
     if (sel.empty()) {
-        *avg_us = *bytes = *flops = 0.0;
+        *avg_us = *bytes = *flops = *voxel_steps = 0.0;
         return SCFT_OK;
     }
     std::swap(sel, c->launches);
@@ -1572,6 +1574,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     *avg_us = 1000.0 * ms / (reps * nl);
     *bytes = B / nl;
     *flops = Fl / nl;
+    *voxel_steps = U / nl;
     return SCFT_OK;
 }
 

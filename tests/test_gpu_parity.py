@@ -239,3 +239,32 @@ def test_c2_full_size_matches_golden(abi, configs, lib, zkind):
         ctx.close()
     finally:
         os.environ.pop("SCFT_ZKIND", None)
+
+
+def test_c2_fp32_split_batch_matches_golden(abi, configs, lib):
+    """The served path (batch of 3 C2 slots, FP32-split propagators, one
+    hipGraph per iteration) against the oracle's FP64 fixture: F, RMSDev and
+    ||dw|| within 1e-7 relative, the fields within 1e-6 of their norm."""
+    g = _golden()["c2_droplet_seed0"]
+    p = configs.params_c2()
+    b = abi.Batch(lib, p, 3, precision=abi.SCFT_FP32_SPLIT)
+    print(b.schedule())
+    w, pf = configs.droplet_fields(p, radius=6.0, seed=0)
+    for s in range(3):
+        b.init_slot(s, p)
+        b.set_fields(s, w, pf)
+        b.set_mask(s, configs.brush_mask(p))
+        b.activate(s)
+    for row in g["iters"]:
+        outs = b.iterate([row["iter"]] * 3)
+        for s in range(3):
+            o = outs[s]
+            assert _scalar_rel(o.free_energy, row["F"]) <= 1e-7, (s, row["iter"], o.free_energy, row["F"])
+            assert _scalar_rel(o.rmsdev, row["rmsdev"]) <= 1e-6
+            assert _scalar_rel(o.dw_rms, row["dw_rms"]) <= 1e-5
+    idx = g["sample_idx"]
+    for s in range(3):
+        wf, pfin = b.get_fields(s)
+        for f, vals in g["w_samples"].items():
+            assert np.abs(wf[int(f)][idx] - np.asarray(vals)).max() <= 1e-6 * g["w_l2"][f] / np.sqrt(len(wf[0]))
+    b.close()
