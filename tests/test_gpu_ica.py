@@ -1,0 +1,48 @@
+"""ICA phase-diagram workflow (README.md:59-128) on a small film: the same
+keep/replace decisions and free energies within 1e-6 in FP64 and in the
+FP32-split mode."""
+import json
+import os
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CHI_AB = (12.0, 16.0)
+CHI_AC = (20.0, 28.0)
+
+
+def _run(abi, lib, tmp_path, precision):
+    import ica
+    import serve
+    base = dict(lx=12.0, lz=10.0, dr=1.0, ns=21, vol_dbk=0.2, vol_s1=0.05, len_s1=0.15, g1=0.02)
+    w = ica.ICA(lib, CHI_AB, CHI_AC, str(tmp_path), precision=precision, max_slots=64, max_iters=60000,
+                base=base, delta=8.0, seed=lambda pt: (serve.SEED_DROPLET, {"radius": 4.0, "seed": 0}))
+    w.run(max_passes=3)
+    w.close()
+    return w
+
+
+def test_ica_decisions_fp64_vs_fp32_split(abi, lib, tmp_path):
+    r64 = _run(abi, lib, tmp_path / "fp64", abi.SCFT_FP64)
+    r32 = _run(abi, lib, tmp_path / "fp32", abi.SCFT_FP32_SPLIT)
+    for r in (r64, r32):
+        assert all(pt.status == "converged" for pt in r.points.values())
+        assert r.passes >= 1
+    print("passes", r64.passes, r32.passes, "decisions", len(r64.decisions),
+          "replaced", sum(d[-1] for d in r64.decisions))
+    assert [(d[0], d[1], d[2], d[3], d[6]) for d in r64.decisions] == \
+           [(d[0], d[1], d[2], d[3], d[6]) for d in r32.decisions]
+    for k, pt in r64.points.items():
+        assert abs(pt.F - r32.points[k].F) <= 1e-6, (k, pt.F, r32.points[k].F)
+    for d64, d32 in zip(r64.decisions, r32.decisions):
+        if d64[4] is not None:
+            assert abs(d64[4] - d32[4]) <= 1e-6
+    # one record per point, with the replaced states in its history
+    recs = [json.load(open(os.path.join(str(tmp_path / "fp64"), f)))
+            for f in os.listdir(str(tmp_path / "fp64")) if f.endswith(".json")]
+    assert len(recs) == 3 * len(CHI_AB) * len(CHI_AC)
+    nrep = sum(len(x["history"]) for x in recs)
+    assert nrep == sum(d[6] for d in r64.decisions)
+    for x in recs:
+        assert os.path.exists(os.path.join(str(tmp_path / "fp64"), x["field"]))
