@@ -10,6 +10,12 @@
  *
  * Replaces SCFT.c:82-1664.  Environment: SCFT_PRECISION=fp32 selects the
  * FP32-split propagators (default FP64, the reference's arithmetic).
+ * SCFT_EXACT_RESTART=1 also writes restart_fields.bin beside every
+ * restart_fields.dat: the fields in full precision (the text file keeps the
+ * reference's 6 decimals, SCFT.c:1645-1646).  A restart (counter != 0)
+ * reads the binary file instead of the text when it exists and belongs to
+ * the counter's iteration and this grid, so a resumed run continues
+ * bit-exactly.  No counterpart in the reference.
  *
  * The reference draws one Gaussian deviate per active field and voxel every
  * iteration, also when MF = 0 (SCFT.c:2103), so restart_random.dat depends
@@ -123,6 +129,35 @@ static void die(const char *what)
 {
     perror(what);
     exit(EXIT_FAILURE);
+}
+
+/* Full-precision restart sidecar: "SCFTRST1", u64 iteration, u64 V, then
+ * w[13][V] and p[V] as FP64. */
+#define EXACT_RESTART "restart_fields.bin"
+
+static void write_exact_restart(unsigned long iter, size_t V, const double *w, const double *pf)
+{
+    FILE *f = fopen(EXACT_RESTART, "wb");
+    if (!f) die(EXACT_RESTART);
+    const unsigned long long hdr[2] = {iter, (unsigned long long)V};
+    if (fwrite("SCFTRST1", 1, 8, f) != 8 || fwrite(hdr, sizeof(hdr), 1, f) != 1 ||
+        fwrite(w, sizeof(double), 13 * V, f) != 13 * V || fwrite(pf, sizeof(double), V, f) != V)
+        die(EXACT_RESTART);
+    fclose(f);
+}
+
+/* 1 when the sidecar holds iteration `iter` of a V-point grid (fields read). */
+static int read_exact_restart(unsigned long iter, size_t V, double *w, double *pf)
+{
+    FILE *f = fopen(EXACT_RESTART, "rb");
+    if (!f) return 0;
+    char magic[8];
+    unsigned long long hdr[2];
+    int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "SCFTRST1", 8) == 0 && fread(hdr, sizeof(hdr), 1, f) == 1 &&
+             hdr[0] == iter && hdr[1] == (unsigned long long)V;
+    if (ok) ok = fread(w, sizeof(double), 13 * V, f) == 13 * V && fread(pf, sizeof(double), V, f) == V;
+    fclose(f);
+    return ok;
 }
 
 static void check(int rc, const char *what)
@@ -241,6 +276,12 @@ int main(void)
         iSeed[1] = (unsigned short)(t + 1);
         iSeed[2] = (unsigned short)(t + 2);
         printf("iSeed new = %hu, %hu, %hu\n\n", iSeed[0], iSeed[1], iSeed[2]);
+    } else if (read_exact_restart(iIter, V, w, pf)) {
+        printf("restart fields: %s (full precision)\n", EXACT_RESTART);
+        fid = fopen(RESTART_RANDOM, "r");
+        if (!fid) die("Error opening restart random file");
+        if (fscanf(fid, "%hu %hu %hu", &iSeed[0], &iSeed[1], &iSeed[2]) != 3) die("restart_random.dat");
+        fclose(fid);
     } else {                                                          /* SCFT.c:844-879 */
         fid = fopen(RESTART_FIELDS, "r");
         if (!fid) die("Error opening restart fields file");
@@ -265,6 +306,8 @@ int main(void)
     check(scft_set_fields(ctx, w, pf), "scft_set_fields");
     write_run_txt(&p, &d, iIter);
 
+    const char *exact_env = getenv("SCFT_EXACT_RESTART");
+    const int exact = exact_env && strcmp(exact_env, "0") != 0;
     const size_t ndraw = (size_t)d.n_active * V;
     double *noise = calloc(ndraw ? ndraw : 1, sizeof(double));
     while (iIter <= ITER) {                                           /* SCFT.c:1088 */
@@ -326,6 +369,7 @@ int main(void)
                         fprintf(fid, "%lf \n", pf[idx]);
                     }
             fclose(fid);
+            if (exact) write_exact_restart(iIter, V, w, pf);
             unsigned short cur[3];
             lcg_get(&gen, cur);
             fid = fopen(RESTART_RANDOM, "w");

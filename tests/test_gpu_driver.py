@@ -76,3 +76,63 @@ def test_driver_files_match_oracle_main(abi, configs, orc, tmp_path, name):
         assert ok, fn
         total_bad += bad
     assert total_bad <= 3, total_bad
+
+
+@pytest.mark.parametrize("name", ["film_small", "all_species"])
+def test_exact_restart_resumes_bit_exactly(abi, configs, tmp_path, name):
+    """With SCFT_EXACT_RESTART=1 the driver writes restart_fields.bin beside
+    each snapshot; a run restarted from the last snapshot continues with the
+    same free_energy.dat / energies.dat lines as the uninterrupted run.  The
+    6-decimal text restart alone does not (the reference's own limitation)."""
+    import shutil
+    import paramsfile
+    exe = os.path.join(BIN, "scft_run_" + name)
+    assert os.path.exists(exe), "driver not built: run __graft_entry__.build()"
+    p, iter_max, d_wrt, d_en = _case(configs, name)
+    w, pf = configs.droplet_fields(p, radius=3.0) if name == "film_small" else \
+        configs.random_fields(p, seed=9, amp=0.4, fields=("A", "B"))
+    mask = configs.brush_mask(p) if p.mask else None
+    full = tmp_path / "full"
+    full.mkdir()
+    paramsfile.write_inputs(str(full), p, w, pf, mask, counter=1, seed=(7, 8, 9))
+    env = dict(os.environ, SCFT_EXACT_RESTART="1")
+    r = subprocess.run([exe], cwd=str(full), capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stderr
+    last = int(open(str(full / "counter.dat")).read())
+    assert 1 < last <= iter_max + 1 and os.path.exists(str(full / "restart_fields.bin"))
+    # resume from the last snapshot, with and without the sidecar
+    lines = {}
+    for tag, keep_bin in (("exact", True), ("text", False)):
+        d = tmp_path / tag
+        d.mkdir()
+        for fn in ("counter.dat", "restart_fields.dat", "restart_random.dat", "wMask.dat", "restart_fields.bin"):
+            if os.path.exists(str(full / fn)) and (keep_bin or fn != "restart_fields.bin"):
+                shutil.copy(str(full / fn), str(d / fn))
+        r = subprocess.run([exe], cwd=str(d), capture_output=True, text=True, timeout=120, env=env)
+        assert r.returncode == 0, r.stderr
+        lines[tag] = {fn: open(str(d / fn)).read().splitlines() for fn in ("free_energy.dat", "energies.dat")}
+    ref = {fn: [l for l in open(str(full / fn)).read().splitlines() if int(l.split()[0]) > last]
+           for fn in ("free_energy.dat", "energies.dat")}
+    assert ref["free_energy.dat"], "no iterations after the last snapshot"
+    assert lines["exact"] == ref
+    # the sidecar holds the solver's fields bit for bit: the same iterations
+    # run in-process from the same inputs give identical arrays (no noise here)
+    if p.relaxation == 0:
+        raw = np.fromfile(str(full / "restart_fields.bin"), dtype=np.uint8)
+        assert raw[:8].tobytes() == b"SCFTRST1"
+        it, V = np.frombuffer(raw[8:24].tobytes(), dtype=np.uint64)
+        data = np.frombuffer(raw[24:].tobytes(), dtype=np.float64)
+        assert it == last and data.size == 14 * V
+        ctx = abi.Context(lib_for_driver(abi), p, abi.SCFT_FP64)
+        ctx.set_fields(w, pf)
+        if mask is not None:
+            ctx.set_mask(mask)
+        for k in range(2, last + 1):
+            ctx.iterate(k)
+        wg, pg = ctx.get_fields()
+        ctx.close()
+        assert np.array_equal(data[:13 * V].reshape(13, V), wg) and np.array_equal(data[13 * V:], pg)
+
+
+def lib_for_driver(abi):
+    return abi.load_library()
