@@ -858,21 +858,35 @@ __global__ void __launch_bounds__(64 * kFftWaves) k_fftwave(const LineOp *__rest
 }
 
 
-// ------------------------------------- periodic FFT of length 256, in registers
+// ------------------------- periodic FFT of length L = 16 P, in registers
 //
-// Four-step transform, 256 = 16 x 16: lane p of a 16-lane group holds
-// x[p + 16 m] (m = 0..15) of one complex line (two real grid lines), does a
-// 16-point DFT over m in registers, multiplies by w_256^{p k2}, and one LDS
-// transpose gives lane q the column k2 = q, whose 16-point DFT over p yields
-// X[q + 16 k1].  K is applied in that order and the inverse runs the same
+// Four-step transform, L = 16 x P (P = 8: L = 128; P = 16: L = 256).  P
+// lanes hold one complex line (two real grid lines): lane p holds
+// x[p + P m] (m = 0..15), does a 16-point DFT over m in registers and
+// multiplies by w_L^{p k2}; one LDS transpose gives lane q the columns
+// c = q + P r (r < 16/P) with their P values, whose P-point DFTs yield
+// X[c + 16 k1].  K is applied in that order and the inverse runs the same
 // steps backwards, so the spectrum is never reordered.  Each line makes two
 // LDS passes (one transpose per direction) instead of one per radix pass.
-// A workgroup of 4 waves takes 16 complex lines = 32 consecutive real lines:
-// for strided (x, y) lines one grid row of those 32 lines is one 128-byte
-// segment, loaded as point pairs of two adjacent lines.
+// A wave holds G = 64/P complex lines; lanes are line-minor (lane = g + G p)
+// so the lanes that load one grid row of strided (x, y) lines are adjacent
+// and a quarter-wave touches few cache lines.  Waves are persistent over
+// units (the 2G real lines of one wave) and the next unit's loads are in
+// flight during the current unit's transforms.
 
-constexpr int kF16Waves = 4;
-constexpr int kF16Lines = 2 * 4 * kF16Waves;   // real lines per workgroup
+constexpr int kFRWaves = 4;
+
+template <int P>
+struct FReg {
+    static constexpr int L = 16 * P;
+    static constexpr int G = 64 / P;           // complex lines per wave
+    static constexpr int R = 16 / P;           // columns per lane after the transpose
+    static constexpr int RS = P + 1;           // tile row stride (complex)
+    // line tiles G16 = 16/G slots (of 16 B) apart in the bank rows: the G
+    // lines x 16/G points of one 16-lane b128 phase hit disjoint banks
+    static constexpr int RG = 16 * RS + 16 / G;
+    static constexpr int UNIT = 2 * G;         // real lines per wave unit
+};
 
 // w_16^{m} = exp(SIGN 2 pi i m / 16)
 template <int SIGN>
@@ -884,14 +898,14 @@ __device__ __forceinline__ cpx<double> w16(int m)
     return cpx<double>{C[m & 15], SIGN * S[m & 15]};
 }
 
-// In-place 16-point DFT with w = exp(SIGN 2 pi i / 16), natural order in and out.
-template <int SIGN>
-__device__ __forceinline__ void dft16(cpx<double> (&v)[16])
+// In-place 16-point DFT of v[o .. o+15] with w = exp(SIGN 2 pi i / 16).
+template <int SIGN, int N>
+__device__ __forceinline__ void dft16(cpx<double> (&v)[N], int o = 0)
 {
     cpx<double> t[4][4];
 #pragma unroll
     for (int n1 = 0; n1 < 4; ++n1) {
-        cpx<double> a[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+        cpx<double> a[4] = {v[o + n1], v[o + n1 + 4], v[o + n1 + 8], v[o + n1 + 12]};
         dft_small<4>(a, double(SIGN));
 #pragma unroll
         for (int k2 = 0; k2 < 4; ++k2) t[n1][k2] = (n1 * k2 == 0) ? a[k2] : cmul(a[k2], w16<SIGN>(n1 * k2));
@@ -901,30 +915,55 @@ __device__ __forceinline__ void dft16(cpx<double> (&v)[16])
         cpx<double> a[4] = {t[0][k2], t[1][k2], t[2][k2], t[3][k2]};
         dft_small<4>(a, double(SIGN));
 #pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = a[k1];
+        for (int k1 = 0; k1 < 4; ++k1) v[o + k2 + 4 * k1] = a[k1];
     }
 }
 
+// In-place 8-point DFT of v[o .. o+7]: radix 4 over n2 (n = n1 + 2 n2), w_8
+// twiddles, radix 2 over n1.
+template <int SIGN, int N>
+__device__ __forceinline__ void dft8(cpx<double> (&v)[N], int o)
+{
+    cpx<double> t[2][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 2; ++n1) {
+        cpx<double> a[4] = {v[o + n1], v[o + n1 + 2], v[o + n1 + 4], v[o + n1 + 6]};
+        dft_small<4>(a, double(SIGN));
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) t[n1][k2] = (n1 * k2 == 0) ? a[k2] : cmul(a[k2], w16<SIGN>(2 * n1 * k2));
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+        v[o + k2] = cadd(t[0][k2], t[1][k2]);
+        v[o + k2 + 4] = csub(t[0][k2], t[1][k2]);
+    }
+}
 
+template <int P, int SIGN, int N>
+__device__ __forceinline__ void dft_p(cpx<double> (&v)[N], int o)
+{
+    if constexpr (P == 16) dft16<SIGN>(v, o);
+    else dft8<SIGN>(v, o);
+}
 
-// Loads of one unit (4 complex lines of a wave) held until the unit starts.
+// Loads of one unit held until the unit starts.
 template <typename Tin>
-struct F16Regs {
+struct FRegLoads {
     Pair<Tin> x[16];
 };
 
-// Unit u of the launch: op u / upo, real lines 8 (u % upo) + 2 g, +1.
-template <typename Tin>
-__device__ __forceinline__ void f16_fetch(const LineOp *__restrict__ ops, int upo, int u, int nunits, int g, int p,
-                                          F16Regs<Tin> &R)
+// Unit u of the launch: op u / upo, real lines UNIT (u % upo) + 2 g, +1.
+template <int P, typename Tin>
+__device__ __forceinline__ void freg_fetch(const LineOp *__restrict__ ops, int upo, int u, int nunits, int g, int p,
+                                           FRegLoads<Tin> &R)
 {
     u = min(u, nunits - 1);
     const LineOp &op = ops[u / upo];
-    const int r0 = (u % upo) * 8 + 2 * g;
+    const int r0 = (u % upo) * FReg<P>::UNIT + 2 * g;
     const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
     const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
     const Tin *src = static_cast<const Tin *>(op.src);
-    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = 16u * unsigned(op.stride);
+    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = unsigned(P) * unsigned(op.stride);
     if (op.pairs) {
 #pragma unroll
         for (int m = 0; m < 16; ++m) R.x[m] = ldg_pair(src, o0 + m * step);
@@ -935,48 +974,39 @@ __device__ __forceinline__ void f16_fetch(const LineOp *__restrict__ ops, int up
     }
 }
 
-// Persistent: each wave strides over units; the next unit's loads are in
-// flight during the current unit's transforms.  The 256 twiddles sit in LDS
-// once per workgroup (every op of a launch has the same length).
-template <typename Tin, typename Tout>
-__global__ void __launch_bounds__(64 * kF16Waves) k_fft256(const LineOp *__restrict__ ops, int nunits, int upo)
+template <int P, typename Tin, typename Tout>
+__global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restrict__ ops, int nunits, int upo)
 {
-    // LDS banks: a b128 access is served 16 lanes at a time, i.e. 4 lines (g)
-    // x 4 points (p).  Rows of a line's 16x16 tile are RS = 17 complex apart
-    // (columns spread over the banks) and line tiles RG = 16 RS + 4 apart,
-    // so the 4 lines of a phase start 16 banks apart.  The twiddle table has
-    // one pad entry per 16.
-    constexpr int L = 256, RS = 17, RG = 16 * RS + 4;
-    __shared__ cpx<double> T[kF16Waves * 4 * RG];
-    __shared__ cpx<double> TW[L + L / 16];
+    using F = FReg<P>;
+    constexpr int L = F::L, G = F::G, R = F::R, RS = F::RS, RG = F::RG;
+    __shared__ cpx<double> T[kFRWaves * G * RG];
+    __shared__ cpx<double> TW[L + L / 16];   // w_L^{-e} at e + e/16 (one pad per 16 entries)
     const int tid = threadIdx.x, lane = tid & 63;
     {
         const double *twd = reinterpret_cast<const double *>(ops[0].tw);
-        for (int e = tid; e < L; e += 64 * kF16Waves)
+        for (int e = tid; e < L; e += 64 * kFRWaves)
             TW[e + (e >> 4)] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
     }
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    // line-minor lanes: the four lanes of one grid row are adjacent, so a
-    // quarter-wave touches 4 cache lines per load instead of 16
-    const int g = lane & 3, p = lane >> 2;
-    cpx<double> *Tg = T + (wave * 4 + g) * RG;
-    const int ustride = gridDim.x * kF16Waves;
-    int u = blockIdx.x * kF16Waves + wave;
+    const int g = lane % G, p = lane / G;   // line-minor
+    cpx<double> *Tg = T + (wave * G + g) * RG;
+    const int ustride = gridDim.x * kFRWaves;
+    int u = blockIdx.x * kFRWaves + wave;
     if (u >= nunits) return;   // no workgroup barrier follows
-    F16Regs<Tin> R;
-    f16_fetch<Tin>(ops, upo, u, nunits, g, p, R);
+    FRegLoads<Tin> ld;
+    freg_fetch<P, Tin>(ops, upo, u, nunits, g, p, ld);
     for (; u < nunits; u += ustride) {
         const LineOp &op = ops[u / upo];
-        const int r0 = (u % upo) * 8 + 2 * g;
+        const int r0 = (u % upo) * F::UNIT + 2 * g;
         const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
         const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
         const int stride = op.stride;
         cpx<double> v[16];
 #pragma unroll
-        for (int m = 0; m < 16; ++m) v[m] = cpx<double>{double(R.x[m].x), double(R.x[m].y)};
-        f16_fetch<Tin>(ops, upo, u + ustride, nunits, g, p, R);   // in flight during the transforms
-        const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(stride), step = 16u * unsigned(stride);
+        for (int m = 0; m < 16; ++m) v[m] = cpx<double>{double(ld.x[m].x), double(ld.x[m].y)};
+        freg_fetch<P, Tin>(ops, upo, u + ustride, nunits, g, p, ld);   // in flight during the transforms
+        const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(stride), step = unsigned(P) * unsigned(stride);
         const unsigned o1 = o0 + unsigned(b1 - b0);
         if (op.epre) {   // first axis of a step (periodic z): not prefetched
 #pragma unroll
@@ -985,7 +1015,7 @@ __global__ void __launch_bounds__(64 * kF16Waves) k_fft256(const LineOp *__restr
                 v[m].y *= pre_factor(op.epre, op.epre2, o1 + m * step);
             }
         }
-        // forward: over m, twiddle w^{-p k2}, transpose, over p
+        // forward: over m, twiddle w^{-p k2}, transpose, over p (per column)
         dft16<-1>(v);
 #pragma unroll
         for (int k = 1; k < 16; ++k) {
@@ -998,30 +1028,39 @@ __global__ void __launch_bounds__(64 * kF16Waves) k_fft256(const LineOp *__restr
         wave_sync();
         const int q = p;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = Tg[q * RS + k];
-        dft16<-1>(v);   // v[k1] = X[q + 16 k1]
+        for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) {
-            const double kk = ldg(op.K, q + 16 * k1) * (1.0 / L);   // 1/L folded in (exact)
-            v[k1].x *= kk;
-            v[k1].y *= kk;
-        }
-        // inverse: over k1 (index p out), twiddle w^{+p q}, transpose, over q
-        dft16<+1>(v);
+            for (int k = 0; k < P; ++k) v[r * P + k] = Tg[(q + P * r) * RS + k];
 #pragma unroll
-        for (int k = 1; k < 16; ++k) {
-            const int e = (q * k) & (L - 1);
-            cpx<double> w = TW[e + (e >> 4)];
-            w.y = -w.y;
-            v[k] = cmul(v[k], w);
+        for (int r = 0; r < R; ++r) {
+            dft_p<P, -1>(v, r * P);   // v[r P + k1] = X[c + 16 k1], c = q + P r
+#pragma unroll
+            for (int k1 = 0; k1 < P; ++k1) {
+                const double kk = ldg(op.K, unsigned(q + P * r + 16 * k1)) * (1.0 / L);   // 1/L folded in (exact)
+                v[r * P + k1].x *= kk;
+                v[r * P + k1].y *= kk;
+            }
+            dft_p<P, +1>(v, r * P);   // over k1, output index p
         }
+        // inverse: twiddle w^{+p c}, transpose back, over k2
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int k = 1; k < P; ++k) {
+                const int e = (k * (q + P * r)) & (L - 1);
+                cpx<double> w = TW[e + (e >> 4)];
+                w.y = -w.y;
+                v[r * P + k] = cmul(v[r * P + k], w);
+            }
         wave_sync();
 #pragma unroll
-        for (int k = 0; k < 16; ++k) Tg[q * RS + k] = v[k];
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int k = 0; k < P; ++k) Tg[(q + P * r) * RS + k] = v[r * P + k];
         wave_sync();
 #pragma unroll
         for (int k = 0; k < 16; ++k) v[k] = Tg[k * RS + p];
-        dft16<+1>(v);   // v[m] = x[p + 16 m] after the operator
+        dft16<+1>(v);   // v[m] = x[p + P m] after the operator
         Tout *dst = static_cast<Tout *>(op.dst);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {

@@ -199,7 +199,7 @@ struct Slot {
     std::vector<QSlot> qslots;
 };
 
-enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFT256 = 5 };
+enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFTREG = 5 };
 constexpr int kLaunchKinds = 6;
 
 struct Launch {
@@ -234,7 +234,7 @@ struct scft_ctx {
     std::vector<std::unique_ptr<DenseTable>> denses;
     int zkind = Z_FFT;
     bool fft_wave = true;            // wave-per-line FFT for supported periodic lengths
-    bool fft_reg = true;             // four-step register FFT for periodic length 256
+    bool fft_reg = true;             // four-step register FFT for periodic lengths 128 and 256
     bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
     bool dense_mfma1 = false;        // force the one-wave-per-SIMD MFMA kernel for even NZ too
     bool tmp_wide = false;           // FP64 scratch between axis operators in FP32 mode
@@ -789,12 +789,13 @@ int build_schedule(scft_ctx *c)
                     }
                 ln.count = int(lops.size()) - ln.first;
                 ln.L = L;
-                if (c->fft_reg && !dirichlet && L == 256) {
-                    // persistent waves over (op, 8-line unit); two workgroups per CU
-                    ln.kind = LK_FFT256;
-                    ln.P = (maxl + 7) / 8;   // units per op
+                if (c->fft_reg && !dirichlet && (L == 128 || L == 256)) {
+                    // persistent waves over (op, unit of 2G real lines); two workgroups per CU
+                    ln.kind = LK_FFTREG;
+                    const int unit = (L == 256) ? FReg<16>::UNIT : FReg<8>::UNIT;
+                    ln.P = (maxl + unit - 1) / unit;   // units per op
                     const int units = ln.count * ln.P;
-                    ln.gx = std::max(1, std::min((units + kF16Waves - 1) / kF16Waves, 2 * c->ncu));
+                    ln.gx = std::max(1, std::min((units + kFRWaves - 1) / kFRWaves, 2 * c->ncu));
                 } else if (c->fft_wave && !dirichlet && fftwave_supported(L)) {
                     ln.kind = LK_FFTWAVE;
                     ln.gx = (maxl + 2 * kFftWaves - 1) / (2 * kFftWaves);
@@ -929,9 +930,13 @@ void launch_one(scft_ctx *c, const Launch &ln)
     case LK_FFTWAVE:
         launch_fftwave<Tin, Tout>(c, ln);
         break;
-    case LK_FFT256:
-        hipLaunchKernelGGL((k_fft256<Tin, Tout>), dim3(ln.gx), dim3(64 * kF16Waves), 0, st, c->lineops + ln.first,
-                           ln.count * ln.P, ln.P);
+    case LK_FFTREG:
+        if (ln.L == 256)
+            hipLaunchKernelGGL((k_fftreg<16, Tin, Tout>), dim3(ln.gx), dim3(64 * kFRWaves), 0, st,
+                               c->lineops + ln.first, ln.count * ln.P, ln.P);
+        else
+            hipLaunchKernelGGL((k_fftreg<8, Tin, Tout>), dim3(ln.gx), dim3(64 * kFRWaves), 0, st,
+                               c->lineops + ln.first, ln.count * ln.P, ln.P);
         break;
     default:
         hipLaunchKernelGGL((k_lineop<Tin, Tout>), g, dim3(kThreads), ln.lds, st, c->lineops + ln.first, ln.P);
@@ -1517,7 +1522,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     const double ts = double(c->tsize);
     for (const Launch &ln : c->launches) {
         const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
-                          ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFT256) && ln.pos == 0);
+                          ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFTREG) && ln.pos == 0);
         if ((kind == 0) != is_z) continue;
         if (kind == 1 && ln.pos != 2 && ln.pos != 3) continue;
         sel.push_back(ln);
@@ -1585,8 +1590,8 @@ int scft_get_schedule(const scft_ctx *c, char *buf, int len)
     for (auto &ln : c->launches)
         if (ln.kind >= 0 && ln.kind < kLaunchKinds) n[ln.kind]++;
     std::snprintf(buf, len,
-                  "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fft256=%d dense_mfma=%d dense_mfma2=%d graph=%d",
-                  c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFT256],
+                  "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fftreg=%d dense_mfma=%d dense_mfma2=%d graph=%d",
+                  c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFTREG],
                   n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0);
     return SCFT_OK;
 }
