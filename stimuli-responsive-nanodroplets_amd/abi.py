@@ -276,6 +276,7 @@ class Batch:
         check(lib, lib.scft_batch_create(C.byref(proto), nslots, max_nx, precision, C.byref(h)),
               "scft_batch_create")
         self.h = h
+        self.max_nx = max(max_nx, grid_of(proto)[0])
         self.V = [0] * nslots
         self.params = [None] * nslots
         self._outs = (ScftIterOut * nslots)()

@@ -21,6 +21,7 @@ import time
 import numpy as np
 
 import abi
+from abi import grid_of
 
 SEED_DROPLET = "droplet"
 SEED_LAMELLA = "lamella"
@@ -204,3 +205,100 @@ def percentiles(values, qs=(50, 99)):
         return {q: None for q in qs}
     v = np.sort(np.asarray(values, float))
     return {q: float(np.percentile(v, q)) for q in qs}
+
+
+def structure_key(p):
+    """Requests whose parameters share this key share one chain structure
+    and can be slots of one batch (the library's same_structure rule: NS,
+    MASK, LY, LZ, DR, whether there is an x axis, and per species whether
+    it is present, its length and composition; LX and chi may differ)."""
+    k = [p.ns, p.mask, p.ly, p.lz, p.dr_target, p.lx == 1.0]
+    for i in range(4):
+        k.append((p.vol_dbk[i] > 0, p.len_dbk[i] if p.vol_dbk[i] > 0 else 0.0,
+                  p.f_dbk[i] if p.vol_dbk[i] > 0 else 0.0))
+    for i in range(10):
+        k.append((p.vol_homo[i] > 0, p.len_homo[i] if p.vol_homo[i] > 0 else 0.0))
+    for i in range(3):
+        k.append((p.vol_s[i] > 0, p.len_s[i] if p.vol_s[i] > 0 else 0.0))
+    return tuple(k)
+
+
+class Server:
+    """Continuous batching over requests of several chain structures (e.g.
+    2D and 3D trial solves in one stream): one Pool per structure, created
+    on the structure's first request and sized against the HBM still free
+    then (capped by `slots`, a dict structure-key -> slots, or
+    `default_slots`).  Requests with an arrival time in the future wait in
+    `pending` until the server's clock reaches it (run_stream)."""
+
+    def __init__(self, lib, precision=abi.SCFT_FP32_SPLIT, default_slots=192, slots=None, max_nx=0,
+                 hbm_fraction=0.85, tol=1e-5, max_iters=10 ** 6):
+        self.lib = lib
+        self.precision = precision
+        self.default_slots = default_slots
+        self.slots = slots or {}
+        self.max_nx = max_nx
+        self.hbm_fraction = hbm_fraction
+        self.tol = tol
+        self.max_iters = max_iters
+        self.pools = {}
+        self.pending = []         # (arrival, seq, request) not yet arrived
+        self.rejected = []
+        self._seq = 0
+        self.t0 = None
+
+    def pool_for(self, req):
+        key = structure_key(req.params)
+        pool = self.pools.get(key)
+        if pool is None:
+            nx = grid_of(req.params)[0]
+            pool = Pool(self.lib, req.params, precision=self.precision, max_slots=self.slots.get(key, self.default_slots),
+                        max_nx=max(self.max_nx, nx), hbm_fraction=self.hbm_fraction, tol=self.tol,
+                        max_iters=self.max_iters)
+            self.pools[key] = pool
+        return pool
+
+    def submit(self, req):
+        pool = self.pool_for(req)
+        if grid_of(req.params)[0] > pool.batch.max_nx:
+            req.status = "rejected"
+            self.rejected.append(req)
+            return
+        pool.submit(req)
+
+    def busy(self):
+        return bool(self.pending) or any(p.busy() for p in self.pools.values())
+
+    def step(self):
+        n = 0
+        for pool in self.pools.values():
+            if pool.busy():
+                n += pool.step()
+        return n
+
+    def done(self):
+        return [r for p in self.pools.values() for r in p.done]
+
+    def run_stream(self, requests, deadline=None):
+        """Serve requests whose `arrival` is a time offset (s) from now: each
+        is submitted when the clock passes it, while earlier ones run."""
+        import heapq
+        self.t0 = time.perf_counter()
+        for r in requests:
+            heapq.heappush(self.pending, (r.arrival, self._seq, r))
+            self._seq += 1
+        while self.busy():
+            now = time.perf_counter() - self.t0
+            while self.pending and self.pending[0][0] <= now:
+                _, _, r = heapq.heappop(self.pending)
+                r.arrival = self.t0 + r.arrival      # absolute, as Pool.submit expects
+                self.submit(r)
+            if deadline is not None and now > deadline:
+                break
+            if self.step() == 0 and self.pending:
+                time.sleep(min(0.001, max(0.0, self.pending[0][0] - now)))
+
+    def close(self):
+        for p in self.pools.values():
+            p.close()
+        self.pools = {}
