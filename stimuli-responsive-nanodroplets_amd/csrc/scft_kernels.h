@@ -3,19 +3,25 @@
 // Propagator: q_{s+1} = E . M (E . q_s) with E = exp(-DS w / 2) and
 // M = F^-1 diag(K) F / (NX NY N) (SCFT.c:2474-2631).  K is separable,
 // K = Kx Ky Kz, so M factors into one real line operator per axis, applied
-// to every grid line of that axis:
-//   - k_lineop: LDS-resident Stockham FFT -> K -> inverse FFT over a tile of
-//     lines (two real lines packed per complex line).  With MASK 1 the z
-//     lines are odd-extended to N = 2(NZ+1) first, which is the reference's
-//     Dirichlet transform (SCFT.c:2551-2592).
-//   - k_zdense: the same z operator as an NZ x NZ matrix.  M commutes with
-//     the reversal k -> NZ-1-k, so it splits into a symmetric and an
-//     antisymmetric half of (NZ/2)^2 each: NZ^2/2 FMA per line, against
-//     the FFT of length 2(NZ+1) = 2*3*43 (NZ = 128) whose prime factor 43
-//     costs ~6x that.  The autotuner picks per size.
-// All transform arithmetic is FP64; propagator slices are stored as T
-// (double, or float for the FP32-split mode).  Every launch covers every
-// live chain of every active instance of the batch (grid.y = op index).
+// to every grid line of that axis, z first.  Slices are stored without the
+// factor of the step that produced them; the first axis of the next step
+// applies it together with its own (see pre_factor), and the readers of
+// slices (Q, contour sums, slice read-back) apply it on read.
+//   - k_zdense_mfma2 / k_zdense_mfma / k_zdense: the z operator as an
+//     NZ x NZ matrix.  M commutes with the reversal k -> NZ-1-k, so it
+//     splits into a symmetric and an antisymmetric half of (NZ/2)^2 each:
+//     NZ^2/2 FMA per line on FP64 MFMA (VALU for NZ > 128), against the
+//     FFT of length 2(NZ+1) = 2*3*43 (NZ = 128) whose prime factor 43 costs
+//     ~6x that.  The autotuner picks per size.
+//   - k_fftreg<P>: periodic lines of length 16 P (128, 256) as a four-step
+//     FFT in registers; k_fftwave: other compile-time lengths, one
+//     wavefront per line; k_lineop: any length, LDS Stockham passes, and the
+//     reference's Dirichlet transform (odd extension to N = 2(NZ+1),
+//     SCFT.c:2551-2592) when the dense z operator is not used.
+// All transform arithmetic is FP64; propagator slices (and the scratch
+// between axes) are stored as T (double, or float for the FP32-split
+// mode).  Every launch covers every live chain of every active instance of
+// the batch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -313,17 +319,6 @@ constexpr int mfma_ldm(int hsp)
     return hsp + 1;
 }
 
-template <typename Tin>
-__device__ __forceinline__ void load4(const Tin *p, double (&v)[4])
-{
-    if constexpr (sizeof(Tin) == 4) {
-        const float4 f = *reinterpret_cast<const float4 *>(p);
-        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
-    } else {
-        const double2 a = *reinterpret_cast<const double2 *>(p);
-        const double2 b = *reinterpret_cast<const double2 *>(p + 2);
-        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
-    }
 }
 
 // Wave tiles of 16 rows.  A tile is loaded with coalesced row loads (lanes
