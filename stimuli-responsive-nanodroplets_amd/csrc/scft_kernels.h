@@ -319,8 +319,6 @@ constexpr int mfma_ldm(int hsp)
     return hsp + 1;
 }
 
-}
-
 // Wave tiles of 16 rows.  A tile is loaded with coalesced row loads (lanes
 // along z), staged in LDS with the Boltzmann factors applied, and the
 // symmetric/antisymmetric A fragments are read from LDS: the MFMA operand
