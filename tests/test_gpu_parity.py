@@ -268,3 +268,35 @@ def test_c2_fp32_split_batch_matches_golden(abi, configs, lib):
         for f, vals in g["w_samples"].items():
             assert np.abs(wf[int(f)][idx] - np.asarray(vals)).max() <= 1e-6 * g["w_l2"][f] / np.sqrt(len(wf[0]))
     b.close()
+
+
+def test_c1_full_run_matches_golden(abi, configs, lib):
+    """C1 (64x1x64 AB melt, chi N 20, G1 0.1) for all of its 2000 iterations
+    against the oracle's FP64 trajectory (tests/golden/make_golden.py c1):
+    F, RMSDev and -ln Q every 10 iterations and the final fields, 1e-10."""
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "c1_oracle.json")) as f:
+        g = json.load(f)["c1_random_seed0"]
+    p = configs.params_c1()
+    ctx = abi.Context(lib, p, abi.SCFT_FP64)
+    w, pf = configs.random_fields(p, seed=0, amp=0.5)
+    ctx.set_fields(w, pf)
+    rows = {r["iter"]: r for r in g["iters"]}
+    worst = 0.0
+    for it in range(2, g["n"] + 2):
+        o = ctx.iterate(it)
+        r = rows.get(it)
+        if r is None:
+            continue
+        worst = max(worst, _scalar_rel(o.free_energy, r["F"]))
+        assert _scalar_rel(o.free_energy, r["F"]) <= TOL, (it, o.free_energy, r["F"])
+        assert _scalar_rel(o.rmsdev, r["rmsdev"]) <= TOL, it
+        assert _scalar_rel(o.chempot[0], r["chempot"][0]) <= TOL, it
+    print("C1 2000 iterations: worst relative F deviation %.2e" % worst)
+    wf, pfin = ctx.get_fields()
+    idx = g["sample_idx"]
+    for f, vals in g["w_samples"].items():
+        assert _rel(wf[int(f)][idx], vals) <= TOL
+    ctx.close()
