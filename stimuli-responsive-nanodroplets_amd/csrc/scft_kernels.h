@@ -519,113 +519,130 @@ struct Mfma2 {
     static constexpr int XS = 2 * HSP + 2;   // even: rows stay 16-byte aligned
 };
 
-template <typename Tin, typename Tout, int HSP>
-__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp *__restrict__ ops, int ntiles, int tpo)
+// Ms^T / Ma^T of the op into LDS, zero-padded to HSP x HSP (row stride LDM).
+template <int HSP>
+__device__ __forceinline__ void mfma2_stage_ops(const DenseOp &op, double *Ms, double *Ma, int tid, int nthreads)
 {
-    constexpr int NT = Mfma2<HSP>::NT, NK = Mfma2<HSP>::NK, LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
-    __shared__ double Ms[HSP * LDM];
-    __shared__ double Ma[HSP * LDM];
-    __shared__ double X[kMfma2Waves * 8 * XS];
-    const int n = ops[0].n, hs = ops[0].hs;   // n = 2 hs
-    const int tid = threadIdx.x;
-    for (int e = tid; e < HSP * HSP; e += 64 * kMfma2Waves) {
+    constexpr int LDM = Mfma2<HSP>::LDM;
+    const int hs = op.hs;
+    for (int e = tid; e < HSP * HSP; e += nthreads) {
         const int j = e / HSP, k = e - j * HSP;
         const bool in = j < hs && k < hs;
-        Ms[j * LDM + k] = in ? ldg(ops[0].MsT, unsigned(j * ops[0].ldm + k)) : 0.0;
-        Ma[j * LDM + k] = in ? ldg(ops[0].MaT, unsigned(j * ops[0].ldm + k)) : 0.0;
+        Ms[j * LDM + k] = in ? ldg(op.MsT, unsigned(j * op.ldm + k)) : 0.0;
+        Ma[j * LDM + k] = in ? ldg(op.MaT, unsigned(j * op.ldm + k)) : 0.0;
     }
-    __syncthreads();
-    const int lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: tile indices stay scalar
+}
+
+// One 16-row tile (rows rbase..rbase+15 of op) of the z operator, result to
+// dst (the op's output, or a caller's scratch with the same layout).  Ms/Ma
+// are the staged operator halves, Xw the wave's 8-row staging rows.
+template <typename Tin, typename Tout, int HSP>
+__device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *dst, const double *Ms,
+                                           const double *Ma, double *Xw, int lane)
+{
+    constexpr int NT = Mfma2<HSP>::NT, NK = Mfma2<HSP>::NK, LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
+    const int n = op.n, hs = op.hs;   // n = 2 hs
     const int li = lane & 15, lj = lane >> 4;
-    double *Xw = X + wave * 8 * XS;
     const int c = 2 * lane;                 // this lane's point pair in a row
     const bool cl = c < 2 * HSP;            // lanes that own staged columns
     const int cc = c + 1 < n ? c : 0;       // load column (pairs past the row read column 0)
+    const int nr = min(16, op.nrows - rbase);
+    const Tin *src = static_cast<const Tin *>(op.src);
+    // factor: E alone (start slice), E^2 inside a block, E E2 at a block boundary
+    const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? 2 : 3);
+    Pair<Tin> xa[16];
+    Pair<double> ea[16];
+    const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(cc);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
+        xa[r] = ldg_pair(src, a);
+        ea[r] = ldg_pair(op.epre, a);
+    }
+    double sf[NT], af[NT];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        wave_sync();   // the previous reads of the staging rows are done
+        if (cl) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int R = 8 * h + r;
+                double e0 = ea[R].x, e1 = ea[R].y;
+                if (mode == 2) {
+                    e0 *= e0;
+                    e1 *= e1;
+                } else if (mode == 3) {   // one step per block: load the second factor here
+                    const Pair<double> f = ldg_pair(op.epre2, o0 + unsigned(R < nr ? R : 0) * unsigned(n));
+                    e0 *= f.x;
+                    e1 *= f.y;
+                }
+                double2 v;
+                v.x = double(xa[R].x) * e0;
+                v.y = double(xa[R].y) * e1;
+                *reinterpret_cast<double2 *>(&Xw[r * XS + c]) = v;
+            }
+        }
+        wave_sync();
+        if ((li >> 3) == h) {
+            const double *row = Xw + (li & 7) * XS;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                const int j = lj * NT + k;   // j >= hs meets zero rows of Ms^T / Ma^T
+                const double x1 = row[j], x2 = row[max(n - 1 - j, 0)];
+                sf[k] = x1 + x2;
+                af[k] = x1 - x2;
+            }
+        }
+    }
+    d4_t U[NK], W[NK];
+#pragma unroll
+    for (int kt = 0; kt < NK; ++kt) {
+        U[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
+        W[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const double *ms = Ms + (lj * NT + k) * LDM + li;
+        const double *ma = Ma + (lj * NT + k) * LDM + li;
+#pragma unroll
+        for (int kt = 0; kt < NK; ++kt) {
+            U[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[k], ms[16 * kt], U[kt], 0, 0, 0);
+            W[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[k], ma[16 * kt], W[kt], 0, 0, 0);
+        }
+    }
+    // D layout: register r of lane l is row l/16 + 4r, output point l%16
+    // of the column tile; the mirrored half goes to n-1-k
+#pragma unroll
+    for (int kt = 0; kt < NK; ++kt) {
+        const int k = 16 * kt + li;
+        if (k >= hs) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int rr = lj + 4 * r;
+            if (rr >= nr) continue;
+            const unsigned row = unsigned(rbase + rr) * unsigned(n);
+            const double u = U[kt][r], v = W[kt][r];
+            stg(dst, row + k, Tout(u + v));
+            stg(dst, row + n - 1 - k, Tout(u - v));
+        }
+    }
+}
+
+template <typename Tin, typename Tout, int HSP>
+__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp *__restrict__ ops, int ntiles, int tpo)
+{
+    constexpr int LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
+    __shared__ double Ms[HSP * LDM];
+    __shared__ double Ma[HSP * LDM];
+    __shared__ double X[kMfma2Waves * 8 * XS];
+    mfma2_stage_ops<HSP>(ops[0], Ms, Ma, threadIdx.x, 64 * kMfma2Waves);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: tile indices stay scalar
+    double *Xw = X + wave * 8 * XS;
     for (int t = blockIdx.x * kMfma2Waves + wave; t < ntiles; t += gridDim.x * kMfma2Waves) {
         const DenseOp &op = ops[t / tpo];
-        const int rbase = (t % tpo) * 16;
-        const int nr = min(16, op.nrows - rbase);
-        const Tin *src = static_cast<const Tin *>(op.src);
-        // factor: E alone (start slice), E^2 inside a block, E E2 at a block boundary
-        const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? 2 : 3);
-        Pair<Tin> xa[16];
-        Pair<double> ea[16];
-        const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(cc);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
-            xa[r] = ldg_pair(src, a);
-            ea[r] = ldg_pair(op.epre, a);
-        }
-        double sf[NT], af[NT];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            wave_sync();   // the previous reads of the staging rows are done
-            if (cl) {
-#pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const int R = 8 * h + r;
-                    double e0 = ea[R].x, e1 = ea[R].y;
-                    if (mode == 2) {
-                        e0 *= e0;
-                        e1 *= e1;
-                    } else if (mode == 3) {   // one step per block: load the second factor here
-                        const Pair<double> f = ldg_pair(op.epre2, o0 + unsigned(R < nr ? R : 0) * unsigned(n));
-                        e0 *= f.x;
-                        e1 *= f.y;
-                    }
-                    double2 v;
-                    v.x = double(xa[R].x) * e0;
-                    v.y = double(xa[R].y) * e1;
-                    *reinterpret_cast<double2 *>(&Xw[r * XS + c]) = v;
-                }
-            }
-            wave_sync();
-            if ((li >> 3) == h) {
-                const double *row = Xw + (li & 7) * XS;
-#pragma unroll
-                for (int k = 0; k < NT; ++k) {
-                    const int j = lj * NT + k;   // j >= hs meets zero rows of Ms^T / Ma^T
-                    const double x1 = row[j], x2 = row[max(n - 1 - j, 0)];
-                    sf[k] = x1 + x2;
-                    af[k] = x1 - x2;
-                }
-            }
-        }
-        d4_t U[NK], W[NK];
-#pragma unroll
-        for (int kt = 0; kt < NK; ++kt) {
-            U[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
-            W[kt] = d4_t{0.0, 0.0, 0.0, 0.0};
-        }
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            const double *ms = Ms + (lj * NT + k) * LDM + li;
-            const double *ma = Ma + (lj * NT + k) * LDM + li;
-#pragma unroll
-            for (int kt = 0; kt < NK; ++kt) {
-                U[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[k], ms[16 * kt], U[kt], 0, 0, 0);
-                W[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[k], ma[16 * kt], W[kt], 0, 0, 0);
-            }
-        }
-        // D layout: register r of lane l is row l/16 + 4r, output point l%16
-        // of the column tile; the mirrored half goes to n-1-k
-        Tout *dst = static_cast<Tout *>(op.dst);
-#pragma unroll
-        for (int kt = 0; kt < NK; ++kt) {
-            const int k = 16 * kt + li;
-            if (k >= hs) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int rr = lj + 4 * r;
-                if (rr >= nr) continue;
-                const unsigned row = unsigned(rbase + rr) * unsigned(n);
-                const double u = U[kt][r], v = W[kt][r];
-                stg(dst, row + k, Tout(u + v));
-                stg(dst, row + n - 1 - k, Tout(u - v));
-            }
-        }
+        mfma2_tile<Tin, Tout, HSP>(op, (t % tpo) * 16, static_cast<Tout *>(op.dst), Ms, Ma, Xw, lane);
     }
 }
 
@@ -967,19 +984,107 @@ __device__ __forceinline__ void freg_fetch(const LineOp *__restrict__ ops, int u
     }
 }
 
+// The twiddles w_L^{-e} of the op's length into LDS at e + e/16.
+template <int P>
+__device__ __forceinline__ void freg_stage_twiddles(const LineOp &op, cpx<double> *TW, int tid, int nthreads)
+{
+    constexpr int L = FReg<P>::L;
+    const double *twd = reinterpret_cast<const double *>(op.tw);
+    for (int e = tid; e < L; e += nthreads) TW[e + (e >> 4)] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
+}
+
+// The line operator of one unit: lane (g, p) holds x[p + P m] of its
+// complex line (real lines r0, r0 + 1) in v; the result goes to op.dst.
+template <int P, typename Tout>
+__device__ __forceinline__ void freg_apply(const LineOp &op, int r0, cpx<double> (&v)[16], cpx<double> *Tg,
+                                           const cpx<double> *TW, int p)
+{
+    using F = FReg<P>;
+    constexpr int L = F::L, R = F::R, RS = F::RS;
+    const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
+    const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
+    const int stride = op.stride;
+    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(stride), step = unsigned(P) * unsigned(stride);
+    const unsigned o1 = o0 + unsigned(b1 - b0);
+    if (op.epre) {   // first axis of a step (periodic z): not prefetched
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            v[m].x *= pre_factor(op.epre, op.epre2, o0 + m * step);
+            v[m].y *= pre_factor(op.epre, op.epre2, o1 + m * step);
+        }
+    }
+    // forward: over m, twiddle w^{-p k2}, transpose, over p (per column)
+    dft16<-1>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+        const int e = (p * k) & (L - 1);
+        v[k] = cmul(v[k], TW[e + (e >> 4)]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) Tg[k * RS + p] = v[k];
+    wave_sync();
+    const int q = p;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < P; ++k) v[r * P + k] = Tg[(q + P * r) * RS + k];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        dft_p<P, -1>(v, r * P);   // v[r P + k1] = X[c + 16 k1], c = q + P r
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) {
+            const double kk = ldg(op.K, unsigned(q + P * r + 16 * k1)) * (1.0 / L);   // 1/L folded in (exact)
+            v[r * P + k1].x *= kk;
+            v[r * P + k1].y *= kk;
+        }
+        dft_p<P, +1>(v, r * P);   // over k1, output index p
+    }
+    // inverse: twiddle w^{+p c}, transpose back, over k2
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 1; k < P; ++k) {
+            const int e = (k * (q + P * r)) & (L - 1);
+            cpx<double> w = TW[e + (e >> 4)];
+            w.y = -w.y;
+            v[r * P + k] = cmul(v[r * P + k], w);
+        }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < P; ++k) Tg[(q + P * r) * RS + k] = v[r * P + k];
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = Tg[k * RS + p];
+    dft16<+1>(v);   // v[m] = x[p + P m] after the operator
+    Tout *dst = static_cast<Tout *>(op.dst);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        double y0 = v[m].x, y1 = v[m].y;
+        if (op.epost) {
+            y0 *= ldg(op.epost, o0 + m * step);
+            y1 *= ldg(op.epost, o1 + m * step);
+        }
+        if (op.pairs) {
+            if (ok0) stg_pair(dst, o0 + m * step, Tout(y0), Tout(y1));
+        } else {
+            if (ok0) stg(dst, o0 + m * step, Tout(y0));
+            if (ok1) stg(dst, o1 + m * step, Tout(y1));
+        }
+    }
+}
+
 template <int P, typename Tin, typename Tout>
 __global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restrict__ ops, int nunits, int upo)
 {
     using F = FReg<P>;
-    constexpr int L = F::L, G = F::G, R = F::R, RS = F::RS, RG = F::RG;
+    constexpr int L = F::L, G = F::G, RG = F::RG;
     __shared__ cpx<double> T[kFRWaves * G * RG];
     __shared__ cpx<double> TW[L + L / 16];   // w_L^{-e} at e + e/16 (one pad per 16 entries)
     const int tid = threadIdx.x, lane = tid & 63;
-    {
-        const double *twd = reinterpret_cast<const double *>(ops[0].tw);
-        for (int e = tid; e < L; e += 64 * kFRWaves)
-            TW[e + (e >> 4)] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
-    }
+    freg_stage_twiddles<P>(ops[0], TW, tid, 64 * kFRWaves);
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane % G, p = lane / G;   // line-minor
@@ -992,83 +1097,11 @@ __global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restri
     for (; u < nunits; u += ustride) {
         const LineOp &op = ops[u / upo];
         const int r0 = (u % upo) * F::UNIT + 2 * g;
-        const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
-        const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
-        const int stride = op.stride;
         cpx<double> v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) v[m] = cpx<double>{double(ld.x[m].x), double(ld.x[m].y)};
         freg_fetch<P, Tin>(ops, upo, u + ustride, nunits, g, p, ld);   // in flight during the transforms
-        const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(stride), step = unsigned(P) * unsigned(stride);
-        const unsigned o1 = o0 + unsigned(b1 - b0);
-        if (op.epre) {   // first axis of a step (periodic z): not prefetched
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                v[m].x *= pre_factor(op.epre, op.epre2, o0 + m * step);
-                v[m].y *= pre_factor(op.epre, op.epre2, o1 + m * step);
-            }
-        }
-        // forward: over m, twiddle w^{-p k2}, transpose, over p (per column)
-        dft16<-1>(v);
-#pragma unroll
-        for (int k = 1; k < 16; ++k) {
-            const int e = (p * k) & (L - 1);
-            v[k] = cmul(v[k], TW[e + (e >> 4)]);
-        }
-        wave_sync();
-#pragma unroll
-        for (int k = 0; k < 16; ++k) Tg[k * RS + p] = v[k];
-        wave_sync();
-        const int q = p;
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int k = 0; k < P; ++k) v[r * P + k] = Tg[(q + P * r) * RS + k];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            dft_p<P, -1>(v, r * P);   // v[r P + k1] = X[c + 16 k1], c = q + P r
-#pragma unroll
-            for (int k1 = 0; k1 < P; ++k1) {
-                const double kk = ldg(op.K, unsigned(q + P * r + 16 * k1)) * (1.0 / L);   // 1/L folded in (exact)
-                v[r * P + k1].x *= kk;
-                v[r * P + k1].y *= kk;
-            }
-            dft_p<P, +1>(v, r * P);   // over k1, output index p
-        }
-        // inverse: twiddle w^{+p c}, transpose back, over k2
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int k = 1; k < P; ++k) {
-                const int e = (k * (q + P * r)) & (L - 1);
-                cpx<double> w = TW[e + (e >> 4)];
-                w.y = -w.y;
-                v[r * P + k] = cmul(v[r * P + k], w);
-            }
-        wave_sync();
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int k = 0; k < P; ++k) Tg[(q + P * r) * RS + k] = v[r * P + k];
-        wave_sync();
-#pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = Tg[k * RS + p];
-        dft16<+1>(v);   // v[m] = x[p + P m] after the operator
-        Tout *dst = static_cast<Tout *>(op.dst);
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            double y0 = v[m].x, y1 = v[m].y;
-            if (op.epost) {
-                y0 *= ldg(op.epost, o0 + m * step);
-                y1 *= ldg(op.epost, o1 + m * step);
-            }
-            if (op.pairs) {
-                if (ok0) stg_pair(dst, o0 + m * step, Tout(y0), Tout(y1));
-            } else {
-                if (ok0) stg(dst, o0 + m * step, Tout(y0));
-                if (ok1) stg(dst, o1 + m * step, Tout(y1));
-            }
-        }
+        freg_apply<P, Tout>(op, r0, v, Tg, TW, p);
     }
 }
 
