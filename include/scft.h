@@ -33,7 +33,8 @@ enum {
     SCFT_ENOMEM      = -3,  /* device allocation failed */
     SCFT_EVOLUME     = -4,  /* free-chain volumes do not sum to 1 (SCFT.c:84-101) */
     SCFT_EUNSUPPORTED= -5,  /* grid/transform size outside what the kernels support */
-    SCFT_ENONFINITE  = -6   /* a NaN/Inf appeared in a reduction */
+    SCFT_ENONFINITE  = -6,  /* a NaN/Inf appeared in a reduction */
+    SCFT_ERNG        = -7   /* device noise stream: too few accepted candidate pairs (see scft_rng_seed) */
 };
 
 enum {
@@ -142,6 +143,22 @@ int scft_update_fields(scft_ctx *ctx, unsigned long iter, float mf,
 /* One whole iteration: propagate + densities + update (SCFT.c:1088-1580). */
 int scft_iterate(scft_ctx *ctx, unsigned long iter, float mf,
                  const double *noise, scft_iter_out *out);
+
+/*
+ * Device replay of the noise stream.  The reference draws one deviate per
+ * active field and voxel every iteration, also when MF = 0 (funGaussDev,
+ * SCFT.c:1769-1782, called at 2103), from drand48 seeded by seed48 or
+ * restart_random.dat (SCFT.c:789-797, 870-879) and saved to
+ * restart_random.dat (SCFT.c:1652-1663).  scft_rng_seed starts the same
+ * stream on the GPU from a seed48 state (xsubi[0] = low 16 bits); from then
+ * on scft_iterate / scft_update_fields take noise = NULL, draw the
+ * iteration's deviates on the device in the reference's order and add
+ * them when mf != 0.  scft_rng_state returns the state after the last
+ * completed iteration (what restart_random.dat holds).  Single-instance
+ * contexts only.  Replaces the host-side drand48 loop of SCFT.c:2103.
+ */
+int scft_rng_seed(scft_ctx *ctx, const unsigned short xsubi[3]);
+int scft_rng_state(scft_ctx *ctx, unsigned short xsubi[3]);
 
 /*
  * Batches.  A batch context holds nslots independent SCFT instances that

@@ -41,6 +41,7 @@ def _load(abi):
         "orc_get_seed": ([vp, P(C.c_ushort)], None),
         "orc_drand48": ([vp], C.c_double),
         "orc_gauss": ([vp], C.c_double),
+        "orc_gauss_n": ([vp, C.c_long, C.POINTER(C.c_double)], None),
         "orc_iterate": ([vp, C.c_ulong, P(abi.ScftIterOut)], C.c_int),
         "orc_pde_block": ([vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, C.c_ulong], None),
         "orc_main": ([P(abi.ScftParams), C.c_ulong, C.c_int, C.c_int, C.c_char_p], C.c_int),
@@ -152,6 +153,11 @@ class OracleState:
 
     def gauss(self):
         return self.lib.orc_gauss(self.h)
+
+    def gauss_n(self, n):
+        out = np.empty(n)
+        self.lib.orc_gauss_n(self.h, n, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
 
     def iterate(self, it):
         o = self.abi.ScftIterOut()

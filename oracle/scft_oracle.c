@@ -458,6 +458,16 @@ double orc_gauss(orc_state *st)
     return x1 * w;
 }
 
+/* n successive deviates of the stream (what one iteration draws when n =
+ * n_active * V), into out, or only advancing the state when out is NULL. */
+void orc_gauss_n(orc_state *st, long n, double *out)
+{
+    for (long i = 0; i < n; i++) {
+        const double g = orc_gauss(st);
+        if (out) out[i] = g;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Propagator: funPDESolver_periodic (SCFT.c:2474-2525) and             */
 /* funPDESolver_DirichletBC (SCFT.c:2527-2631).                         */

@@ -41,6 +41,7 @@ void orc_seed48(orc_state *st, const unsigned short seed[3]);
 void orc_get_seed(const orc_state *st, unsigned short seed[3]);
 double orc_drand48(orc_state *st);
 double orc_gauss(orc_state *st);     /* funGaussDev, SCFT.c:1769-1782 */
+void orc_gauss_n(orc_state *st, long n, double *out);   /* n successive deviates (out may be NULL) */
 
 /* One iteration of SCFT.c:1090-1580; iter is the value after iIter++. */
 int  orc_iterate(orc_state *st, unsigned long iter, scft_iter_out *out);

@@ -18,7 +18,8 @@ SCFT_FP64 = 0
 SCFT_FP32_SPLIT = 1
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "out of device memory",
-          -4: "volumes do not sum to 1", -5: "unsupported size", -6: "non-finite value"}
+          -4: "volumes do not sum to 1", -5: "unsupported size", -6: "non-finite value",
+          -7: "device noise stream short of accepted pairs"}
 
 
 class ScftParams(C.Structure):
@@ -150,6 +151,8 @@ def load_library(path=None):
         "scft_densities": ([vp, dp, dp], C.c_int),
         "scft_update_fields": ([vp, C.c_ulong, C.c_float, dp, P(ScftIterOut)], C.c_int),
         "scft_iterate": ([vp, C.c_ulong, C.c_float, dp, P(ScftIterOut)], C.c_int),
+        "scft_rng_seed": ([vp, P(C.c_ushort)], C.c_int),
+        "scft_rng_state": ([vp, P(C.c_ushort)], C.c_int),
         "scft_sync": ([vp], C.c_int),
         "scft_last_error": ([], C.c_char_p),
         "scft_batch_create": ([P(ScftParams), C.c_int, C.c_int, C.c_int, P(vp)], C.c_int),
@@ -255,6 +258,17 @@ class Context:
         nz = None if noise is None else _ptr(np.ascontiguousarray(noise, dtype=np.float64))
         check(self.lib, self.lib.scft_iterate(self.h, it, mf, nz, C.byref(o)), "scft_iterate")
         return o
+
+    def rng_seed(self, xsubi):
+        """Draw the noise stream on the device from this drand48 state
+        (seed48 layout); iterate() then takes no noise array."""
+        x = (C.c_ushort * 3)(*xsubi)
+        check(self.lib, self.lib.scft_rng_seed(self.h, x), "scft_rng_seed")
+
+    def rng_state(self):
+        x = (C.c_ushort * 3)()
+        check(self.lib, self.lib.scft_rng_state(self.h, x), "scft_rng_state")
+        return tuple(x)
 
     def sync(self):
         check(self.lib, self.lib.scft_sync(self.h), "scft_sync")

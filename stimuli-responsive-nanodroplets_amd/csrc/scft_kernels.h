@@ -1105,6 +1105,132 @@ __global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restri
     }
 }
 
+// ------------------------------------------------------------ noise stream
+//
+// The reference's deviates (funGaussDev, SCFT.c:1769-1782): POSIX drand48,
+// x <- (a x + c) mod 2^48 with a = 0x5DEECE66D, c = 0xB, u = x / 2^48; a
+// deviate takes uniform pairs (u1, u2) until w = v1^2 + v2^2 < 1 with
+// v = 2u - 1, then returns v1 sqrt(-2 ln w / w) and drops the rest.  One
+// iteration draws need = n_active * V deviates (SCFT.c:2103), consuming a
+// random number of pairs.  In parallel: candidate pair m of the iteration
+// uses draws 2m+1 and 2m+2 from the state at its start; chunk t holds
+// pairs [kRngChunk t, kRngChunk (t+1)).  k_rng_count counts accepted pairs
+// per chunk (jumping to the chunk's start), k_rng_scan turns the counts
+// into the deviate index of each chunk's first accepted pair, k_rng_emit
+// writes the deviates in draw order and the state after the pair of the
+// last one, and k_rng_commit makes that the next iteration's state.  The
+// acceptance test is evaluated with the host's IEEE operations (no
+// contraction into FMA), so the accepted pairs, hence the state, are the
+// host's bit for bit; the deviate itself goes through the device log.
+
+constexpr int kRngChunk = 32;   // candidate pairs per thread
+constexpr unsigned long long kLcgA = 0x5DEECE66DULL, kLcgC = 0xBULL, kLcgMask = 0xFFFFFFFFFFFFULL;
+
+struct RngJump {
+    unsigned long long a[48], c[48];   // x -> a[b] x + c[b] advances 2^b draws
+};
+
+struct RngState {
+    unsigned long long x;      // drand48 state at the start of the next iteration
+    unsigned long long xn;     // state after this iteration's last deviate
+    unsigned long long found;  // accepted pairs among the candidates
+    int short_of_pairs;        // the candidates held fewer than `need` accepted pairs
+};
+
+__device__ __forceinline__ unsigned long long lcg_jump(const RngJump &J, unsigned long long x, unsigned long long k)
+{
+    for (int b = 0; k; ++b, k >>= 1)
+        if (k & 1) x = (J.a[b] * x + J.c[b]) & kLcgMask;
+    return x;
+}
+
+// One candidate pair from state x (advanced by two draws); returns whether
+// it is accepted and sets v1, w.
+__device__ __forceinline__ bool polar_pair(unsigned long long &x, double &v1, double &w)
+{
+#pragma clang fp contract(off)
+    x = (kLcgA * x + kLcgC) & kLcgMask;
+    const double u1 = double(x) * (1.0 / 281474976710656.0);
+    x = (kLcgA * x + kLcgC) & kLcgMask;
+    const double u2 = double(x) * (1.0 / 281474976710656.0);
+    v1 = 2.0 * u1 - 1.0;
+    const double v2 = 2.0 * u2 - 1.0;
+    const double a = v1 * v1;
+    const double b = v2 * v2;
+    w = a + b;
+    return w < 1.0;
+}
+
+__global__ void __launch_bounds__(kThreads) k_rng_count(const RngState *__restrict__ st,
+                                                        const RngJump *__restrict__ J, int nchunk,
+                                                        unsigned *__restrict__ counts)
+{
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= nchunk) return;
+    unsigned long long x = lcg_jump(*J, st->x, 2ull * kRngChunk * (unsigned long long)t);
+    unsigned n = 0;
+    for (int m = 0; m < kRngChunk; ++m) {
+        double v1, w;
+        n += polar_pair(x, v1, w) ? 1u : 0u;
+    }
+    counts[t] = n;
+}
+
+// One workgroup: exclusive scan of the chunk counts (each thread a
+// contiguous run of chunks, then a workgroup scan of the run totals).
+__global__ void __launch_bounds__(1024) k_rng_scan(RngState *__restrict__ st, const unsigned *__restrict__ counts,
+                                                   int nchunk, unsigned long long need,
+                                                   unsigned long long *__restrict__ offsets)
+{
+    __shared__ unsigned long long part[1024];
+    const int t = threadIdx.x;
+    const int per = (nchunk + 1023) / 1024;
+    const int b0 = min(t * per, nchunk), b1 = min(b0 + per, nchunk);
+    unsigned long long sum = 0;
+    for (int i = b0; i < b1; ++i) sum += counts[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive Hillis-Steele scan
+        const unsigned long long v = t >= d ? part[t - d] : 0ull;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned long long run = part[t] - sum;
+    for (int i = b0; i < b1; ++i) {
+        offsets[i] = run;
+        run += counts[i];
+    }
+    if (t == 1023) {
+        st->found = part[1023];
+        st->short_of_pairs = part[1023] < need ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_rng_emit(RngState *__restrict__ st, const RngJump *__restrict__ J,
+                                                       int nchunk, const unsigned *__restrict__ counts,
+                                                       const unsigned long long *__restrict__ offsets,
+                                                       unsigned long long need, double *__restrict__ out)
+{
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= nchunk) return;
+    unsigned long long j = offsets[t];
+    if (j >= need || counts[t] == 0) return;
+    unsigned long long x = lcg_jump(*J, st->x, 2ull * kRngChunk * (unsigned long long)t);
+    for (int m = 0; m < kRngChunk && j < need; ++m) {
+        double v1, w;
+        if (!polar_pair(x, v1, w)) continue;
+        out[j] = v1 * sqrt((-2.0 * log(w)) / w);
+        if (j == need - 1) st->xn = x;
+        ++j;
+    }
+}
+
+__global__ void k_rng_commit(RngState *st)
+{
+    if (threadIdx.x == 0 && !st->short_of_pairs) st->x = st->xn;
+}
+
 // ------------------------------------------------------------------ prep
 
 struct PrepArgs {

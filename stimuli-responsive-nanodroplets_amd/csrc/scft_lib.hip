@@ -261,6 +261,15 @@ struct scft_ctx {
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // device replay of the noise stream (single-slot contexts)
+    bool rng_on = false;
+    RngState *rng = nullptr;             // device
+    RngState *rng_host = nullptr;        // pinned copy, refreshed with the scalars
+    RngJump *rng_jump = nullptr;         // device
+    unsigned *rng_counts = nullptr;      // [rng_nchunk]
+    unsigned long long *rng_offsets = nullptr;
+    int rng_nchunk = 0;
+    unsigned long long rng_need = 0;     // deviates per iteration (n_active * V)
 };
 
 namespace {
@@ -972,6 +981,20 @@ void launch_densities(scft_ctx *c, int nact)
     hipLaunchKernelGGL(k_density<T>, dim3(c->nred, nact), dim3(kThreads), 0, c->stream, (const DenArgs *)c->d_den);
 }
 
+// This iteration's deviates into slot 0's noise buffer, in draw order.
+void launch_rng(scft_ctx *c)
+{
+    if (c->rng_need == 0) return;
+    const dim3 g((c->rng_nchunk + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_rng_count, g, dim3(kThreads), 0, c->stream, c->rng, c->rng_jump, c->rng_nchunk,
+                       c->rng_counts);
+    hipLaunchKernelGGL(k_rng_scan, dim3(1), dim3(1024), 0, c->stream, c->rng, c->rng_counts, c->rng_nchunk,
+                       c->rng_need, c->rng_offsets);
+    hipLaunchKernelGGL(k_rng_emit, g, dim3(kThreads), 0, c->stream, c->rng, c->rng_jump, c->rng_nchunk,
+                       c->rng_counts, c->rng_offsets, c->rng_need, c->slots[0].noise);
+    hipLaunchKernelGGL(k_rng_commit, dim3(1), dim3(64), 0, c->stream, c->rng);
+}
+
 void launch_update(scft_ctx *c, int nact)
 {
     hipLaunchKernelGGL(k_update, dim3(c->nred, nact), dim3(kThreads), 0, c->stream, (const UpdArgs *)c->d_upd);
@@ -1001,9 +1024,15 @@ int enqueue(scft_ctx *c, int phases)
         if (f32) launch_densities<float>(c, nact);
         else launch_densities<double>(c, nact);
     }
-    if (phases & 4) launch_update(c, nact);
-    if (phases & 8)
+    if (phases & 4) {
+        if (c->rng_on) launch_rng(c);
+        launch_update(c, nact);
+    }
+    if (phases & 8) {
         HIPCHK(hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scalars) * c->nslots, hipMemcpyDeviceToHost, c->stream));
+        if (c->rng_on)
+            HIPCHK(hipMemcpyAsync(c->rng_host, c->rng, sizeof(RngState), hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(hipGetLastError());
     return SCFT_OK;
 }
@@ -1059,6 +1088,7 @@ void free_ctx(scft_ctx *c)
     drop_graph(c);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
+    if (c->rng_host) (void)hipHostFree(c->rng_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1444,9 +1474,31 @@ int scft_densities(scft_ctx *c, double *qsum, double *chempot)
     return SCFT_OK;
 }
 
+static int check_rng(scft_ctx *c)
+{
+    if (c->rng_on && c->rng_need && c->rng_host->short_of_pairs) {
+        set_err("device noise stream: " + std::to_string(c->rng_host->found) + " accepted pairs among the candidates, " +
+                std::to_string(c->rng_need) + " needed");
+        return SCFT_ERNG;
+    }
+    return SCFT_OK;
+}
+
 static int upload_noise(scft_ctx *c, float mf, const double *noise, const double **dev)
 {
     *dev = nullptr;
+    if (c->rng_on) {   // the device draws the deviates
+        if (noise) {
+            set_err("noise must be NULL while the device noise stream is on");
+            return SCFT_EINVAL;
+        }
+        if (c->act.size() != 1 || c->act[0] != 0) {
+            set_err("the device noise stream needs slot 0 active alone");
+            return SCFT_EINVAL;
+        }
+        if (mf != 0.0f) *dev = c->slots[0].noise;
+        return SCFT_OK;
+    }
     if (mf == 0.0f) return SCFT_OK;
     if (!noise) {
         set_err("noise is required when mf != 0");
@@ -1474,6 +1526,7 @@ int scft_update_fields(scft_ctx *c, unsigned long iter, float mf, const double *
     RETCHK(set_iteration_args(c, mf, nd));
     RETCHK(enqueue(c, 4 | 8));
     HIPCHK(hipStreamSynchronize(c->stream));
+    RETCHK(check_rng(c));
     if (out && !c->act.empty()) fill_out(c, c->act[0], mf, out);
     return SCFT_OK;
 }
@@ -1486,7 +1539,85 @@ int scft_iterate(scft_ctx *c, unsigned long iter, float mf, const double *noise,
     const double *nd = nullptr;
     RETCHK(upload_noise(c, mf, noise, &nd));
     RETCHK(run_iteration(c, mf, nd));
+    RETCHK(check_rng(c));
     if (out && !c->act.empty()) fill_out(c, c->act[0], mf, out);
+    return SCFT_OK;
+}
+
+// Seed the device replay of the noise stream with a drand48 state
+// (seed48 layout: xsubi[0] the low 16 bits).  Sizes the candidate pairs
+// per iteration: need / (pi/4) plus `SCFT_RNG_MARGIN` (default 8) times
+// sqrt(need) + 256 spare pairs, about 17 standard deviations of the
+// accepted count; fewer accepted pairs than needed is reported as
+// SCFT_ERNG and leaves the state unadvanced.
+int scft_rng_seed(scft_ctx *c, const unsigned short xsubi[3])
+{
+    if (!c || !xsubi) return SCFT_EINVAL;
+    if (c->nslots != 1) {
+        set_err("the device noise stream is for single-instance contexts");
+        return SCFT_EINVAL;
+    }
+    Slot &s = c->slots[0];
+    const unsigned long long need = (unsigned long long)s.d.n_active * (unsigned long long)s.V;
+    double k = 8.0;
+    if (const char *m = std::getenv("SCFT_RNG_MARGIN")) k = std::atof(m);
+    const double pairs = (double(need) + k * std::sqrt(double(need)) + (k > 0 ? 256.0 : 0.0)) / (M_PI / 4.0);
+    const unsigned long long nchunk = need ? (unsigned long long)std::ceil(std::max(pairs, 1.0) / kRngChunk) : 0;
+    if (need >= (1ull << 31) || nchunk >= (1ull << 30)) {
+        set_err("noise stream too long for one iteration");
+        return SCFT_EUNSUPPORTED;
+    }
+    if (!c->rng) {
+        RETCHK(ctx_alloc(c, &c->rng, 1));
+        RETCHK(ctx_alloc(c, &c->rng_jump, 1));
+        if (hipHostMalloc(reinterpret_cast<void **>(&c->rng_host), sizeof(RngState), hipHostMallocDefault) !=
+            hipSuccess) {
+            c->rng_host = nullptr;
+            set_err("hipHostMalloc failed");
+            return SCFT_ENOMEM;
+        }
+    }
+    if (int(nchunk) > c->rng_nchunk) {
+        RETCHK(ctx_alloc(c, &c->rng_counts, size_t(nchunk)));
+        RETCHK(ctx_alloc(c, &c->rng_offsets, size_t(nchunk)));
+    }
+    if (!s.noise) RETCHK(ctx_alloc(c, &s.noise, size_t(13) * c->max_V));
+    RngJump J;
+    unsigned long long a = kLcgA, cc = kLcgC;   // x -> a x + cc advances 2^b draws
+    for (int b = 0; b < 48; ++b) {
+        J.a[b] = a;
+        J.c[b] = cc;
+        cc = (a * cc + cc) & kLcgMask;
+        a = (a * a) & kLcgMask;
+    }
+    RngState st{};
+    st.x = ((unsigned long long)xsubi[2] << 32) | ((unsigned long long)xsubi[1] << 16) | xsubi[0];
+    st.xn = st.x;
+    HIPCHK(hipMemcpyAsync(c->rng_jump, &J, sizeof(J), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->rng, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *c->rng_host = st;
+    c->rng_nchunk = int(nchunk);
+    c->rng_need = need;
+    c->rng_on = true;
+    drop_graph(c);   // the iteration now includes the generator
+    return SCFT_OK;
+}
+
+// The drand48 state after the last completed iteration.
+int scft_rng_state(scft_ctx *c, unsigned short xsubi[3])
+{
+    if (!c || !xsubi) return SCFT_EINVAL;
+    if (!c->rng_on) {
+        set_err("the device noise stream is off (scft_rng_seed)");
+        return SCFT_EINVAL;
+    }
+    RngState st;
+    HIPCHK(hipMemcpyAsync(&st, c->rng, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    xsubi[0] = (unsigned short)(st.x & 0xffff);
+    xsubi[1] = (unsigned short)((st.x >> 16) & 0xffff);
+    xsubi[2] = (unsigned short)((st.x >> 32) & 0xffff);
     return SCFT_OK;
 }
 

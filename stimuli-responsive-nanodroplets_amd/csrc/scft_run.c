@@ -19,10 +19,12 @@
  *
  * The reference draws one Gaussian deviate per active field and voxel every
  * iteration, also when MF = 0 (SCFT.c:2103), so restart_random.dat depends
- * on the whole stream.  This driver replays POSIX drand48 (48-bit LCG,
- * a = 0x5DEECE66D, c = 0xB) and the Marsaglia polar method on the host:
- * with MF = 1 the deviates are uploaded; with MF = 0 only the generator
- * state is advanced, on a host thread that overlaps the GPU iteration.
+ * on the whole stream.  By default the library replays that stream on the
+ * GPU (scft_rng_seed / scft_rng_state).  SCFT_HOST_RNG=1 replays it here
+ * instead: POSIX drand48 (48-bit LCG, a = 0x5DEECE66D, c = 0xB) and the
+ * Marsaglia polar method; with MF = 1 the deviates are uploaded, with
+ * MF = 0 only the generator state is advanced, on a host thread that
+ * overlaps the GPU iteration.
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -303,19 +305,24 @@ int main(void)
         fclose(fid);
     }
     lcg_seed(&gen, iSeed);
+    const char *host_rng_env = getenv("SCFT_HOST_RNG");
+    const int host_rng = host_rng_env && strcmp(host_rng_env, "0") != 0;
+    if (!host_rng) check(scft_rng_seed(ctx, iSeed), "scft_rng_seed");
     check(scft_set_fields(ctx, w, pf), "scft_set_fields");
     write_run_txt(&p, &d, iIter);
 
     const char *exact_env = getenv("SCFT_EXACT_RESTART");
     const int exact = exact_env && strcmp(exact_env, "0") != 0;
     const size_t ndraw = (size_t)d.n_active * V;
-    double *noise = calloc(ndraw ? ndraw : 1, sizeof(double));
+    double *noise = host_rng ? calloc(ndraw ? ndraw : 1, sizeof(double)) : NULL;
     while (iIter <= ITER) {                                           /* SCFT.c:1088 */
         iIter++;
         const float MF = (iIter > RELAXATION) ? 0.0f : 1.0f;           /* SCFT.c:1515-1522 */
         scft_iter_out o;
         draw_job job = {&gen, ndraw, NULL};
-        if (MF != 0.0f) {
+        if (!host_rng) {
+            check(scft_iterate(ctx, iIter, MF, NULL, &o), "scft_iterate");
+        } else if (MF != 0.0f) {
             job.out = noise;
             draw_thread(&job);
             check(scft_iterate(ctx, iIter, MF, noise, &o), "scft_iterate");
@@ -371,7 +378,8 @@ int main(void)
             fclose(fid);
             if (exact) write_exact_restart(iIter, V, w, pf);
             unsigned short cur[3];
-            lcg_get(&gen, cur);
+            if (host_rng) lcg_get(&gen, cur);
+            else check(scft_rng_state(ctx, cur), "scft_rng_state");
             fid = fopen(RESTART_RANDOM, "w");
             fprintf(fid, "%hu %hu %hu", cur[0], cur[1], cur[2]);
             fclose(fid);

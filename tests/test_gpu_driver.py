@@ -40,8 +40,11 @@ def _tokens_close(a, b, tol):
     return True, bad
 
 
-@pytest.mark.parametrize("name", ["film_small", "all_species"])
-def test_driver_files_match_oracle_main(abi, configs, orc, tmp_path, name):
+@pytest.mark.parametrize("name,rng", [("film_small", "device"), ("film_small", "host"), ("all_species", "device")])
+def test_driver_files_match_oracle_main(abi, configs, orc, tmp_path, name, rng):
+    """The noise stream (on in film_small's first iterations) drawn on the
+    GPU (default) and replayed on the host (SCFT_HOST_RNG=1) both leave the
+    oracle's restart_random.dat."""
     import paramsfile
     exe = os.path.join(BIN, "scft_run_" + name)
     assert os.path.exists(exe), "driver not built: run __graft_entry__.build()"
@@ -60,7 +63,8 @@ def test_driver_files_match_oracle_main(abi, configs, orc, tmp_path, name):
         paramsfile.write_inputs(str(d), p, w, pf, mask, counter=1, seed=(7, 8, 9))
         dirs[who] = str(d)
     print("running", exe, flush=True)
-    r = subprocess.run([exe], cwd=dirs["gpu"], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ, SCFT_HOST_RNG="1" if rng == "host" else "0")
+    r = subprocess.run([exe], cwd=dirs["gpu"], capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stderr
     assert orc.main(p, iter_max, d_wrt, d_en, dirs["oracle"]) == 0
     names = sorted(os.listdir(dirs["oracle"]))
