@@ -154,8 +154,10 @@ int scft_iterate(scft_ctx *ctx, unsigned long iter, float mf,
  * on scft_iterate / scft_update_fields take noise = NULL, draw the
  * iteration's deviates on the device in the reference's order and add
  * them when mf != 0.  scft_rng_state returns the state after the last
- * completed iteration (what restart_random.dat holds).  Single-instance
- * contexts only.  Replaces the host-side drand48 loop of SCFT.c:2103.
+ * completed iteration (what restart_random.dat holds).  These act on slot
+ * 0; scft_slot_rng_seed / scft_slot_rng_state below give every slot of a
+ * batch its own stream.  Replaces the host-side drand48 loop of
+ * SCFT.c:2103.
  */
 int scft_rng_seed(scft_ctx *ctx, const unsigned short xsubi[3]);
 int scft_rng_state(scft_ctx *ctx, unsigned short xsubi[3]);
@@ -186,9 +188,18 @@ int scft_slot_copy_fields(scft_ctx *ctx, int dst_slot, int src_slot);
 int scft_slot_set_mask(scft_ctx *ctx, int slot, const double *mask);
 int scft_slot_get_densities(scft_ctx *ctx, int slot, double *phi, double *phi_tot);
 int scft_slot_get_propagator(scft_ctx *ctx, int slot, int sp, int which, int s, double *out);
-/* One iteration of every active slot.  iters[slot] (or NULL: no seed box)
- * selects the dbk1 seed start; outs[slot] is filled for active slots.
- * No noise (MF = 0) in batch mode. */
+/* A slot's own noise stream (see scft_rng_seed; no counterpart in the
+ * reference, which runs one instance per process).  A slot with a stream
+ * draws n_active*V deviates every batch iteration and adds them while its
+ * iteration is within its RELAXATION (MF = 1, SCFT.c:1515-1522).
+ * scft_slot_init turns the stream off. */
+int scft_slot_rng_seed(scft_ctx *ctx, int slot, const unsigned short xsubi[3]);
+int scft_slot_rng_state(scft_ctx *ctx, int slot, unsigned short xsubi[3]);
+/* One iteration of every active slot.  iters[slot] (or NULL: no seed box,
+ * no noise) is the slot's iteration counter: it selects the dbk1 seed
+ * start and, for slots with a noise stream, MF.  A slot whose iteration is
+ * within its RELAXATION but that has no stream is an error (the reference
+ * adds noise there).  outs[slot] is filled for active slots. */
 int scft_batch_iterate(scft_ctx *ctx, const unsigned long *iters, scft_iter_out *outs);
 /* Device bytes one slot of a batch with prototype p needs (fields,
  * propagators, scratch); the serving layer admits solves by this footprint. */

@@ -153,6 +153,8 @@ def load_library(path=None):
         "scft_iterate": ([vp, C.c_ulong, C.c_float, dp, P(ScftIterOut)], C.c_int),
         "scft_rng_seed": ([vp, P(C.c_ushort)], C.c_int),
         "scft_rng_state": ([vp, P(C.c_ushort)], C.c_int),
+        "scft_slot_rng_seed": ([vp, C.c_int, P(C.c_ushort)], C.c_int),
+        "scft_slot_rng_state": ([vp, C.c_int, P(C.c_ushort)], C.c_int),
         "scft_sync": ([vp], C.c_int),
         "scft_last_error": ([], C.c_char_p),
         "scft_batch_create": ([P(ScftParams), C.c_int, C.c_int, C.c_int, P(vp)], C.c_int),
@@ -318,6 +320,16 @@ class Batch:
     def activate(self, s, on=True):
         check(self.lib, self.lib.scft_slot_activate(self.h, s, 1 if on else 0), "scft_slot_activate")
 
+    def rng_seed(self, s, xsubi):
+        """Give slot s its own noise stream from this drand48 state."""
+        x = (C.c_ushort * 3)(*xsubi)
+        check(self.lib, self.lib.scft_slot_rng_seed(self.h, s, x), "scft_slot_rng_seed")
+
+    def rng_state(self, s):
+        x = (C.c_ushort * 3)()
+        check(self.lib, self.lib.scft_slot_rng_state(self.h, s, x), "scft_slot_rng_state")
+        return tuple(x)
+
     def set_fields(self, s, w, p):
         w = np.ascontiguousarray(w, dtype=np.float64).reshape(NF, self.V[s])
         p = np.ascontiguousarray(p, dtype=np.float64).reshape(self.V[s])
@@ -349,12 +361,19 @@ class Batch:
               "scft_slot_get_propagator")
         return out
 
+    @property
+    def iteration_numbers(self):
+        """The per-slot iteration-number array iterate() passes (fill in place)."""
+        return self._iters
+
     def iterate(self, iters=None):
-        """One iteration of every active slot; returns the out structs (by slot)."""
+        """One iteration of every active slot; returns the out structs (by slot).
+        iters: per-slot iteration numbers (seed box, MF), or None."""
         ip = None
         if iters is not None:
-            for i, v in enumerate(iters):
-                self._iters[i] = v
+            if iters is not self._iters:
+                for i, v in enumerate(iters):
+                    self._iters[i] = v
             ip = self._iters
         check(self.lib, self.lib.scft_batch_iterate(self.h, ip, self._outs), "scft_batch_iterate")
         return self._outs

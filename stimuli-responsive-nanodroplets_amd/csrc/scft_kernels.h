@@ -1118,7 +1118,9 @@ __global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restri
 // per chunk (jumping to the chunk's start), k_rng_scan turns the counts
 // into the deviate index of each chunk's first accepted pair, k_rng_emit
 // writes the deviates in draw order and the state after the pair of the
-// last one, and k_rng_commit makes that the next iteration's state.  The
+// last one, and k_rng_commit makes that the next iteration's state.  Each
+// instance of a batch has its own stream: grid y runs over the instances
+// that draw (RngSlot).  The
 // acceptance test is evaluated with the host's IEEE operations (no
 // contraction into FMA), so the accepted pairs, hence the state, are the
 // host's bit for bit; the deviate itself goes through the device log.
@@ -1135,6 +1137,16 @@ struct RngState {
     unsigned long long xn;     // state after this iteration's last deviate
     unsigned long long found;  // accepted pairs among the candidates
     int short_of_pairs;        // the candidates held fewer than `need` accepted pairs
+};
+
+// One instance's stream for one iteration.
+struct RngSlot {
+    RngState *st;
+    unsigned *counts;              // [nchunk]
+    unsigned long long *offsets;   // [nchunk]
+    double *out;                   // need deviates in draw order
+    unsigned long long need;
+    int nchunk;
 };
 
 __device__ __forceinline__ unsigned long long lcg_jump(const RngJump &J, unsigned long long x, unsigned long long k)
@@ -1161,27 +1173,31 @@ __device__ __forceinline__ bool polar_pair(unsigned long long &x, double &v1, do
     return w < 1.0;
 }
 
-__global__ void __launch_bounds__(kThreads) k_rng_count(const RngState *__restrict__ st,
-                                                        const RngJump *__restrict__ J, int nchunk,
-                                                        unsigned *__restrict__ counts)
+__global__ void __launch_bounds__(kThreads) k_rng_count(const RngSlot *__restrict__ slots,
+                                                        const RngJump *__restrict__ J)
 {
+    const RngSlot &a = slots[blockIdx.y];
     const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nchunk) return;
-    unsigned long long x = lcg_jump(*J, st->x, 2ull * kRngChunk * (unsigned long long)t);
+    if (t >= a.nchunk) return;
+    unsigned long long x = lcg_jump(*J, a.st->x, 2ull * kRngChunk * (unsigned long long)t);
     unsigned n = 0;
     for (int m = 0; m < kRngChunk; ++m) {
         double v1, w;
         n += polar_pair(x, v1, w) ? 1u : 0u;
     }
-    counts[t] = n;
+    a.counts[t] = n;
 }
 
-// One workgroup: exclusive scan of the chunk counts (each thread a
-// contiguous run of chunks, then a workgroup scan of the run totals).
-__global__ void __launch_bounds__(1024) k_rng_scan(RngState *__restrict__ st, const unsigned *__restrict__ counts,
-                                                   int nchunk, unsigned long long need,
-                                                   unsigned long long *__restrict__ offsets)
+// One workgroup per instance: exclusive scan of the chunk counts (each
+// thread a contiguous run of chunks, then a workgroup scan of the run totals).
+__global__ void __launch_bounds__(1024) k_rng_scan(const RngSlot *__restrict__ slots)
 {
+    const RngSlot &a = slots[blockIdx.x];
+    RngState *st = a.st;
+    const unsigned *counts = a.counts;
+    unsigned long long *offsets = a.offsets;
+    const int nchunk = a.nchunk;
+    const unsigned long long need = a.need;
     __shared__ unsigned long long part[1024];
     const int t = threadIdx.x;
     const int per = (nchunk + 1023) / 1024;
@@ -1207,27 +1223,29 @@ __global__ void __launch_bounds__(1024) k_rng_scan(RngState *__restrict__ st, co
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_rng_emit(RngState *__restrict__ st, const RngJump *__restrict__ J,
-                                                       int nchunk, const unsigned *__restrict__ counts,
-                                                       const unsigned long long *__restrict__ offsets,
-                                                       unsigned long long need, double *__restrict__ out)
+__global__ void __launch_bounds__(kThreads) k_rng_emit(const RngSlot *__restrict__ slots,
+                                                       const RngJump *__restrict__ J)
 {
+    const RngSlot &a = slots[blockIdx.y];
     const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= nchunk) return;
-    unsigned long long j = offsets[t];
-    if (j >= need || counts[t] == 0) return;
-    unsigned long long x = lcg_jump(*J, st->x, 2ull * kRngChunk * (unsigned long long)t);
+    if (t >= a.nchunk) return;
+    const unsigned long long need = a.need;
+    unsigned long long j = a.offsets[t];
+    if (j >= need || a.counts[t] == 0) return;
+    unsigned long long x = lcg_jump(*J, a.st->x, 2ull * kRngChunk * (unsigned long long)t);
     for (int m = 0; m < kRngChunk && j < need; ++m) {
         double v1, w;
         if (!polar_pair(x, v1, w)) continue;
-        out[j] = v1 * sqrt((-2.0 * log(w)) / w);
-        if (j == need - 1) st->xn = x;
+        a.out[j] = v1 * sqrt((-2.0 * log(w)) / w);
+        if (j == need - 1) a.st->xn = x;
         ++j;
     }
 }
 
-__global__ void k_rng_commit(RngState *st)
+// grid: one workgroup per instance
+__global__ void k_rng_commit(const RngSlot *__restrict__ slots)
 {
+    RngState *st = slots[blockIdx.x].st;
     if (threadIdx.x == 0 && !st->short_of_pairs) st->x = st->xn;
 }
 

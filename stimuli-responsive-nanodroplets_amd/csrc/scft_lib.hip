@@ -189,6 +189,12 @@ struct Slot {
     void *qd[SCFT_NSPECIES] = {};
     double *qpart = nullptr, *dpart = nullptr, *upart = nullptr;
     double *noise = nullptr;
+    // device noise stream (scft_slot_rng_seed): candidate chunks and their scan
+    bool rng_on = false;
+    unsigned *rng_counts = nullptr;
+    unsigned long long *rng_offsets = nullptr;
+    int rng_nchunk = 0, rng_cap = 0;
+    unsigned long long rng_need = 0;     // deviates per iteration (n_active * V)
     std::vector<Chain> chains;
     const FftTable *fx = nullptr, *fy = nullptr, *fz = nullptr;
     const DenseTable *dz = nullptr;
@@ -261,15 +267,13 @@ struct scft_ctx {
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    // device replay of the noise stream (single-slot contexts)
-    bool rng_on = false;
-    RngState *rng = nullptr;             // device
-    RngState *rng_host = nullptr;        // pinned copy, refreshed with the scalars
+    // device replay of the noise stream, one drand48 state per slot
+    RngState *rng = nullptr;             // [nslots] device
+    RngState *rng_host = nullptr;        // [nslots] pinned copy, refreshed with the scalars
     RngJump *rng_jump = nullptr;         // device
-    unsigned *rng_counts = nullptr;      // [rng_nchunk]
-    unsigned long long *rng_offsets = nullptr;
-    int rng_nchunk = 0;
-    unsigned long long rng_need = 0;     // deviates per iteration (n_active * V)
+    RngSlot *d_rng = nullptr;            // this schedule's drawing slots
+    size_t d_rng_cap = 0;
+    int nrng = 0, rng_maxchunk = 0;
 };
 
 namespace {
@@ -849,6 +853,16 @@ int build_schedule(scft_ctx *c)
     }
     RETCHK(upload(c, &c->d_qslots, &c->d_qslots_cap, qs));
     c->nqslots = int(qs.size());
+    std::vector<RngSlot> rs;
+    c->rng_maxchunk = 0;
+    for (int si : c->act) {
+        Slot &s = c->slots[si];
+        if (!s.rng_on || s.rng_need == 0) continue;
+        rs.push_back(RngSlot{c->rng + si, s.rng_counts, s.rng_offsets, s.noise, s.rng_need, s.rng_nchunk});
+        c->rng_maxchunk = std::max(c->rng_maxchunk, s.rng_nchunk);
+    }
+    RETCHK(upload(c, &c->d_rng, &c->d_rng_cap, rs));
+    c->nrng = int(rs.size());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->dirty = false;
     return SCFT_OK;
@@ -868,8 +882,10 @@ int set_iteration_args(scft_ctx *c, float mf, const double *noise_dev)
             HIPCHK(hipMemcpyAsync(c->d_prep + a, &s.prep, sizeof(PrepArgs), hipMemcpyHostToDevice, c->stream));
             changed = true;
         }
-        const double amp = s.p.dt_noise * mf;
-        const double *nz = (mf != 0.0f) ? noise_dev : nullptr;
+        // mf < 0: each slot's own MF (1 while its iteration is within RELAXATION)
+        const float smf = mf >= 0.0f ? mf : ((s.rng_on && s.iter <= s.p.relaxation) ? 1.0f : 0.0f);
+        const double amp = s.p.dt_noise * smf;
+        const double *nz = (smf != 0.0f) ? (mf >= 0.0f ? noise_dev : s.noise) : nullptr;
         if (amp != s.upd.amp || nz != s.upd.noise) {
             s.upd.amp = amp;
             s.upd.noise = nz;
@@ -981,18 +997,16 @@ void launch_densities(scft_ctx *c, int nact)
     hipLaunchKernelGGL(k_density<T>, dim3(c->nred, nact), dim3(kThreads), 0, c->stream, (const DenArgs *)c->d_den);
 }
 
-// This iteration's deviates into slot 0's noise buffer, in draw order.
+// This iteration's deviates of every drawing slot into its noise buffer,
+// in draw order.
 void launch_rng(scft_ctx *c)
 {
-    if (c->rng_need == 0) return;
-    const dim3 g((c->rng_nchunk + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_rng_count, g, dim3(kThreads), 0, c->stream, c->rng, c->rng_jump, c->rng_nchunk,
-                       c->rng_counts);
-    hipLaunchKernelGGL(k_rng_scan, dim3(1), dim3(1024), 0, c->stream, c->rng, c->rng_counts, c->rng_nchunk,
-                       c->rng_need, c->rng_offsets);
-    hipLaunchKernelGGL(k_rng_emit, g, dim3(kThreads), 0, c->stream, c->rng, c->rng_jump, c->rng_nchunk,
-                       c->rng_counts, c->rng_offsets, c->rng_need, c->slots[0].noise);
-    hipLaunchKernelGGL(k_rng_commit, dim3(1), dim3(64), 0, c->stream, c->rng);
+    if (c->nrng == 0) return;
+    const dim3 g((c->rng_maxchunk + kThreads - 1) / kThreads, c->nrng);
+    hipLaunchKernelGGL(k_rng_count, g, dim3(kThreads), 0, c->stream, c->d_rng, c->rng_jump);
+    hipLaunchKernelGGL(k_rng_scan, dim3(c->nrng), dim3(1024), 0, c->stream, c->d_rng);
+    hipLaunchKernelGGL(k_rng_emit, g, dim3(kThreads), 0, c->stream, c->d_rng, c->rng_jump);
+    hipLaunchKernelGGL(k_rng_commit, dim3(c->nrng), dim3(64), 0, c->stream, c->d_rng);
 }
 
 void launch_update(scft_ctx *c, int nact)
@@ -1025,13 +1039,14 @@ int enqueue(scft_ctx *c, int phases)
         else launch_densities<double>(c, nact);
     }
     if (phases & 4) {
-        if (c->rng_on) launch_rng(c);
+        launch_rng(c);
         launch_update(c, nact);
     }
     if (phases & 8) {
         HIPCHK(hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scalars) * c->nslots, hipMemcpyDeviceToHost, c->stream));
-        if (c->rng_on)
-            HIPCHK(hipMemcpyAsync(c->rng_host, c->rng, sizeof(RngState), hipMemcpyDeviceToHost, c->stream));
+        if (c->nrng)
+            HIPCHK(hipMemcpyAsync(c->rng_host, c->rng, sizeof(RngState) * c->nslots, hipMemcpyDeviceToHost,
+                                  c->stream));
     }
     HIPCHK(hipGetLastError());
     return SCFT_OK;
@@ -1263,6 +1278,8 @@ int scft_slot_init(scft_ctx *c, int slot, const scft_params *p)
     Slot *s = get_slot(c, slot);
     if (!s || !p) return SCFT_EINVAL;
     s->active = false;
+    s->rng_on = false;   // a new instance starts without a noise stream
+    c->dirty = true;
     RETCHK(slot_init(c, *s, *p));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SCFT_OK;
@@ -1476,10 +1493,14 @@ int scft_densities(scft_ctx *c, double *qsum, double *chempot)
 
 static int check_rng(scft_ctx *c)
 {
-    if (c->rng_on && c->rng_need && c->rng_host->short_of_pairs) {
-        set_err("device noise stream: " + std::to_string(c->rng_host->found) + " accepted pairs among the candidates, " +
-                std::to_string(c->rng_need) + " needed");
-        return SCFT_ERNG;
+    for (int si : c->act) {
+        const Slot &s = c->slots[si];
+        if (s.rng_on && s.rng_need && c->rng_host[si].short_of_pairs) {
+            set_err("device noise stream of slot " + std::to_string(si) + ": " +
+                    std::to_string(c->rng_host[si].found) + " accepted pairs among the candidates, " +
+                    std::to_string(s.rng_need) + " needed");
+            return SCFT_ERNG;
+        }
     }
     return SCFT_OK;
 }
@@ -1487,7 +1508,7 @@ static int check_rng(scft_ctx *c)
 static int upload_noise(scft_ctx *c, float mf, const double *noise, const double **dev)
 {
     *dev = nullptr;
-    if (c->rng_on) {   // the device draws the deviates
+    if (c->slots[0].rng_on) {   // the device draws the deviates
         if (noise) {
             set_err("noise must be NULL while the device noise stream is on");
             return SCFT_EINVAL;
@@ -1544,20 +1565,18 @@ int scft_iterate(scft_ctx *c, unsigned long iter, float mf, const double *noise,
     return SCFT_OK;
 }
 
-// Seed the device replay of the noise stream with a drand48 state
+// Seed the device replay of a slot's noise stream with a drand48 state
 // (seed48 layout: xsubi[0] the low 16 bits).  Sizes the candidate pairs
 // per iteration: need / (pi/4) plus `SCFT_RNG_MARGIN` (default 8) times
 // sqrt(need) + 256 spare pairs, about 17 standard deviations of the
 // accepted count; fewer accepted pairs than needed is reported as
-// SCFT_ERNG and leaves the state unadvanced.
-int scft_rng_seed(scft_ctx *c, const unsigned short xsubi[3])
+// SCFT_ERNG and leaves the state unadvanced.  Re-initialising the slot
+// (scft_slot_init) turns its stream off.
+int scft_slot_rng_seed(scft_ctx *c, int slot, const unsigned short xsubi[3])
 {
-    if (!c || !xsubi) return SCFT_EINVAL;
-    if (c->nslots != 1) {
-        set_err("the device noise stream is for single-instance contexts");
-        return SCFT_EINVAL;
-    }
-    Slot &s = c->slots[0];
+    Slot *sp = get_slot(c, slot);
+    if (!sp || !xsubi || !sp->ready) return SCFT_EINVAL;
+    Slot &s = *sp;
     const unsigned long long need = (unsigned long long)s.d.n_active * (unsigned long long)s.V;
     double k = 8.0;
     if (const char *m = std::getenv("SCFT_RNG_MARGIN")) k = std::atof(m);
@@ -1568,52 +1587,56 @@ int scft_rng_seed(scft_ctx *c, const unsigned short xsubi[3])
         return SCFT_EUNSUPPORTED;
     }
     if (!c->rng) {
-        RETCHK(ctx_alloc(c, &c->rng, 1));
+        RETCHK(ctx_alloc(c, &c->rng, size_t(c->nslots)));
         RETCHK(ctx_alloc(c, &c->rng_jump, 1));
-        if (hipHostMalloc(reinterpret_cast<void **>(&c->rng_host), sizeof(RngState), hipHostMallocDefault) !=
-            hipSuccess) {
+        if (hipHostMalloc(reinterpret_cast<void **>(&c->rng_host), sizeof(RngState) * c->nslots,
+                          hipHostMallocDefault) != hipSuccess) {
             c->rng_host = nullptr;
             set_err("hipHostMalloc failed");
             return SCFT_ENOMEM;
         }
+        std::memset(c->rng_host, 0, sizeof(RngState) * c->nslots);
+        RngJump J;
+        unsigned long long a = kLcgA, cc = kLcgC;   // x -> a x + cc advances 2^b draws
+        for (int b = 0; b < 48; ++b) {
+            J.a[b] = a;
+            J.c[b] = cc;
+            cc = (a * cc + cc) & kLcgMask;
+            a = (a * a) & kLcgMask;
+        }
+        HIPCHK(hipMemcpyAsync(c->rng_jump, &J, sizeof(J), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
-    if (int(nchunk) > c->rng_nchunk) {
-        RETCHK(ctx_alloc(c, &c->rng_counts, size_t(nchunk)));
-        RETCHK(ctx_alloc(c, &c->rng_offsets, size_t(nchunk)));
+    if (int(nchunk) > s.rng_cap) {
+        RETCHK(ctx_alloc(c, &s.rng_counts, size_t(nchunk)));
+        RETCHK(ctx_alloc(c, &s.rng_offsets, size_t(nchunk)));
+        s.rng_cap = int(nchunk);
     }
     if (!s.noise) RETCHK(ctx_alloc(c, &s.noise, size_t(13) * c->max_V));
-    RngJump J;
-    unsigned long long a = kLcgA, cc = kLcgC;   // x -> a x + cc advances 2^b draws
-    for (int b = 0; b < 48; ++b) {
-        J.a[b] = a;
-        J.c[b] = cc;
-        cc = (a * cc + cc) & kLcgMask;
-        a = (a * a) & kLcgMask;
-    }
     RngState st{};
     st.x = ((unsigned long long)xsubi[2] << 32) | ((unsigned long long)xsubi[1] << 16) | xsubi[0];
     st.xn = st.x;
-    HIPCHK(hipMemcpyAsync(c->rng_jump, &J, sizeof(J), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->rng, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->rng + slot, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    *c->rng_host = st;
-    c->rng_nchunk = int(nchunk);
-    c->rng_need = need;
-    c->rng_on = true;
-    drop_graph(c);   // the iteration now includes the generator
+    c->rng_host[slot] = st;
+    s.rng_nchunk = int(nchunk);
+    s.rng_need = need;
+    s.rng_on = true;
+    c->dirty = true;   // the iteration now includes the slot's generator
     return SCFT_OK;
 }
 
-// The drand48 state after the last completed iteration.
-int scft_rng_state(scft_ctx *c, unsigned short xsubi[3])
+// The slot's drand48 state after its last completed iteration.
+int scft_slot_rng_state(scft_ctx *c, int slot, unsigned short xsubi[3])
 {
-    if (!c || !xsubi) return SCFT_EINVAL;
-    if (!c->rng_on) {
-        set_err("the device noise stream is off (scft_rng_seed)");
+    Slot *s = get_slot(c, slot);
+    if (!s || !xsubi) return SCFT_EINVAL;
+    if (!s->rng_on) {
+        set_err("the slot's device noise stream is off (scft_slot_rng_seed)");
         return SCFT_EINVAL;
     }
     RngState st;
-    HIPCHK(hipMemcpyAsync(&st, c->rng, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&st, c->rng + slot, sizeof(st), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     xsubi[0] = (unsigned short)(st.x & 0xffff);
     xsubi[1] = (unsigned short)((st.x >> 16) & 0xffff);
@@ -1621,14 +1644,35 @@ int scft_rng_state(scft_ctx *c, unsigned short xsubi[3])
     return SCFT_OK;
 }
 
+int scft_rng_seed(scft_ctx *c, const unsigned short xsubi[3])
+{
+    return scft_slot_rng_seed(c, 0, xsubi);
+}
+
+int scft_rng_state(scft_ctx *c, unsigned short xsubi[3])
+{
+    return scft_slot_rng_state(c, 0, xsubi);
+}
+
 int scft_batch_iterate(scft_ctx *c, const unsigned long *iters, scft_iter_out *outs)
 {
     if (!c) return SCFT_EINVAL;
     if (c->dirty) RETCHK(build_schedule(c));
-    for (int si : c->act) c->slots[si].iter = iters ? iters[si] : 1ul << 40;
-    RETCHK(run_iteration(c, 0.0f, nullptr));
+    for (int si : c->act) {
+        Slot &s = c->slots[si];
+        s.iter = iters ? iters[si] : 1ul << 40;
+        if (s.iter <= s.p.relaxation && !s.rng_on) {   // MF = 1 needs the slot's stream (SCFT.c:1515-1522)
+            set_err("slot " + std::to_string(si) + " is within RELAXATION but has no noise stream (scft_slot_rng_seed)");
+            return SCFT_EINVAL;
+        }
+    }
+    RETCHK(run_iteration(c, -1.0f, nullptr));
+    RETCHK(check_rng(c));
     if (outs)
-        for (int si : c->act) fill_out(c, si, 0.0f, &outs[si]);
+        for (int si : c->act) {
+            const Slot &s = c->slots[si];
+            fill_out(c, si, (s.rng_on && s.iter <= s.p.relaxation) ? 1.0f : 0.0f, &outs[si]);
+        }
     return SCFT_OK;
 }
 

@@ -12,6 +12,13 @@ replay while the active set is unchanged.
 
 Convergence follows the project's criterion (the reference itself runs a
 fixed ITER): ||w_new - w_old||_rms < tol over the active fields.
+
+Iterations of a request are numbered from 1 as in a fresh reference run
+(SCFT.c:1088-1090): the number selects the dbk1 seed box (ENABLE_SEED) and
+MF.  A request whose parameters have RELAXATION > 0 gets its slot's own
+device noise stream (scft_slot_rng_seed), seeded from `Request.rng` or,
+by default, drand48's initial state; its first RELAXATION iterations add
+the reference's noise.
 """
 import collections
 import ctypes as C
@@ -27,13 +34,14 @@ SEED_DROPLET = "droplet"
 SEED_LAMELLA = "lamella"
 SEED_FIELDS = "fields"
 SEED_FROM = "from"        # another request's converged field
+DRAND48_INITIAL = (0x330E, 0xABCD, 0x1234)   # POSIX drand48 state before any seeding
 
 
 class Request:
     __slots__ = ("rid", "params", "seed", "mask", "arrival", "slot", "admit_t", "done_t", "iters",
-                 "F", "status", "fields", "dw", "history", "keep_fields", "start_iter")
+                 "F", "status", "fields", "dw", "history", "keep_fields", "start_iter", "rng")
 
-    def __init__(self, rid, params, seed, mask=None, arrival=None, keep_fields=False):
+    def __init__(self, rid, params, seed, mask=None, arrival=None, keep_fields=False, rng=None):
         self.rid = rid
         self.params = params
         self.seed = seed              # (kind, arg)
@@ -49,7 +57,8 @@ class Request:
         self.fields = None
         self.history = []
         self.keep_fields = keep_fields
-        self.start_iter = 1
+        self.start_iter = 1           # number of the request's first iteration
+        self.rng = rng                # drand48 state (3 x u16) of its noise stream, or None
 
     @property
     def time_to_converge(self):
@@ -137,6 +146,8 @@ class Pool:
             s = self.free_slots.popleft()
             self.batch.init_slot(s, req.params)
             self._seed_fields(req, s)
+            if req.params.relaxation > 0 or req.rng is not None:
+                self.batch.rng_seed(s, req.rng if req.rng is not None else DRAND48_INITIAL)
             self.batch.activate(s, True)
             req.slot = s
             req.admit_t = time.perf_counter()
@@ -162,8 +173,11 @@ class Pool:
         self._admit()
         if not self.running:
             return 0
+        its = self.batch.iteration_numbers
+        for s, req in self.running.items():
+            its[s] = req.start_iter + req.iters
         t0 = time.perf_counter()
-        outs = self.batch.iterate()
+        outs = self.batch.iterate(its)
         now = time.perf_counter()
         self.t_iterate += now - t0
         self.iterations += 1

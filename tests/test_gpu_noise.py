@@ -118,3 +118,92 @@ def test_host_noise_rejected_while_device_stream_on(abi, configs, lib):
     with pytest.raises(abi.ScftError, match="invalid argument"):
         ctx.iterate(2, 1.0, np.zeros(ctx.derived.n_active * ctx.V))
     ctx.close()
+
+
+def test_batch_slot_streams_match_single_contexts(abi, configs, orc, lib):
+    """Every slot of a batch with its own stream (different seeds and
+    RELAXATION) follows the single-instance context with the same seed, bit
+    for bit, and ends at the oracle generator's state."""
+    ps = [configs.t_film_small(), configs.t_film_small(), configs.t_film_small()]
+    chis = [(25.0, 40.0), (18.0, 30.0), (30.0, 45.0)]
+    relax = [2, 3, 0]
+    seeds = [(1, 2, 3), (7, 8, 9), (0x330E, 0xABCD, 0x1234)]
+    for p, (a, c), r in zip(ps, chis, relax):
+        p.chi[0][1] = p.chi[1][0] = a
+        p.chi[0][2] = p.chi[2][0] = c
+        p.relaxation = r
+        p.dt_noise = 0.1
+    b = abi.Batch(lib, ps[0], 3, max_nx=12, precision=abi.SCFT_FP64)
+    singles, n = [], []
+    for s, p in enumerate(ps):
+        w, pf = configs.droplet_fields(p, radius=3.0, seed=s)
+        m = configs.brush_mask(p)
+        d = b.init_slot(s, p)
+        n.append(d.n_active * b.V[s])
+        b.set_fields(s, w, pf)
+        b.set_mask(s, m)
+        b.rng_seed(s, seeds[s])
+        b.activate(s)
+        c = abi.Context(lib, p, abi.SCFT_FP64)
+        c.set_fields(w, pf)
+        c.set_mask(m)
+        c.rng_seed(seeds[s])
+        singles.append(c)
+    gens = []
+    for s in range(3):
+        g = orc.state(configs.t_film_small())
+        g.seed48(seeds[s])
+        gens.append(g)
+    for it in range(1, 6):
+        outs = b.iterate([it, it, it])
+        for s, c in enumerate(singles):
+            o = c.iterate(it, 1.0 if it <= relax[s] else 0.0)
+            assert outs[s].free_energy == o.free_energy, (it, s, outs[s].free_energy, o.free_energy)
+            assert outs[s].mf == (1.0 if it <= relax[s] else 0.0)
+            gens[s].gauss_n(n[s])
+            assert b.rng_state(s) == gens[s].get_seed() == c.rng_state(), (it, s)
+    for s, c in enumerate(singles):
+        wa, _ = b.get_fields(s)
+        wb, _ = c.get_fields()
+        assert np.array_equal(wa, wb), s
+        c.close()
+    b.close()
+
+
+def test_batch_slot_within_relaxation_needs_a_stream(abi, configs, lib):
+    p = configs.t_film_small()
+    p.relaxation = 2
+    b = abi.Batch(lib, p, 1, precision=abi.SCFT_FP64)
+    b.init_slot(0, p)
+    w, pf = configs.droplet_fields(p, radius=3.0)
+    b.set_fields(0, w, pf)
+    b.set_mask(0, configs.brush_mask(p))
+    b.activate(0)
+    with pytest.raises(abi.ScftError, match="RELAXATION"):
+        b.iterate([1])
+    b.iterate([3])            # past RELAXATION: MF = 0, no stream needed
+    b.close()
+
+
+def test_served_request_with_relaxation_matches_single_context(abi, configs, lib):
+    """serve.Pool gives a request with RELAXATION > 0 its own stream (drand48's
+    initial state by default) and numbers its iterations from 1: its
+    free-energy trajectory equals a single context run the same way."""
+    import serve
+    p = configs.t_film_small()
+    p.relaxation = 3
+    p.dt_noise = 0.1
+    w, pf = configs.droplet_fields(p, radius=3.0, seed=4)
+    pool = serve.Pool(lib, p, precision=abi.SCFT_FP64, max_slots=2, tol=1e-30, max_iters=6, record_every=1)
+    req = serve.Request("r", p, (serve.SEED_FIELDS, (w, pf)))
+    pool.submit(req)
+    pool.run()
+    assert req.status == "max_iters"
+    c = abi.Context(lib, p, abi.SCFT_FP64)
+    c.set_fields(w, pf)
+    c.set_mask(configs.brush_mask(p))
+    c.rng_seed(serve.DRAND48_INITIAL)
+    ref = [c.iterate(it, 1.0 if it <= 3 else 0.0).free_energy for it in range(1, 7)]
+    assert [h[1] for h in req.history] == ref
+    c.close()
+    pool.close()
