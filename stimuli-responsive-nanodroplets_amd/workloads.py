@@ -1,0 +1,88 @@
+"""Timing runs of the reference's workloads on one GPU, outside the C2
+bench: per batched iteration and per launch of the z operator and the
+last-axis transform (HIP events, scft_time_launches).
+
+    python workloads.py c3 [--iters N] [--precision fp32|fp64]
+        3D thin-film droplet, 128x128x64, NS 201 (C3 chemistry), one solve
+    python workloads.py c4 [--slots B] [--iters N] [--precision ...]
+        phase-diagram sweep batch: B slots of the C2 chemistry with
+        chi_AB N over 15..36 and chi_AC N over 20..55 (8 x 8 grid), LX drawn
+        per slot from {24, 32, 40} R_g (seeded), so grids are 192/256/320 x
+        1 x 128 of uneven size in one batch
+
+Prints one line per workload.  Fields start from the droplet seed; the
+timed iterations follow two warm-up iterations.
+"""
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+import abi  # noqa: E402
+import configs  # noqa: E402
+
+
+def c3_params():
+    return [configs.params_c3()]
+
+
+def c4_params(slots, seed=0):
+    rng = np.random.default_rng(seed)
+    lx = rng.choice([24.0, 32.0, 40.0], size=slots)
+    chi_ab = np.linspace(15.0, 36.0, 8)
+    chi_ac = np.linspace(20.0, 55.0, 8)
+    out = []
+    for s in range(slots):
+        i, j = (s // 8) % 8, s % 8
+        out.append(configs.chem_droplet(lx=float(lx[s]), chi_ab=float(chi_ab[i]), chi_ac=float(chi_ac[j])))
+    return out
+
+
+def run(lib, name, protos, precision, iters):
+    maxnx = max(abi.grid_of(p)[0] for p in protos)
+    b = abi.Batch(lib, protos[0], len(protos), max_nx=maxnx, precision=precision)
+    for s, p in enumerate(protos):
+        b.init_slot(s, p)
+        w, pf = configs.droplet_fields(p, radius=6.0, seed=s)
+        b.set_fields(s, w, pf)
+        b.set_mask(s, configs.brush_mask(p))
+        b.activate(s)
+    b.iterate()
+    b.iterate()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        outs = b.iterate()
+    dt = (time.perf_counter() - t0) / iters
+    z = b.time_launches(0, 5)
+    x = b.time_launches(1, 5)
+    print("%s slots=%d %s: %.3f ms per batched iteration, %.1f us per instance-iteration; "
+          "z launch %.1f us x%d (%.0f GB/s), last-axis launch %.1f us x%d; F[0]=%.10f; %s"
+          % (name, len(protos), "fp32-split" if precision == abi.SCFT_FP32_SPLIT else "fp64", dt * 1e3,
+             dt / len(protos) * 1e6, z["avg_us"], z["launches"], z["bytes"] / max(z["avg_us"], 1e-9) * 1e-3,
+             x["avg_us"], x["launches"], outs[0].free_energy, b.schedule()), flush=True)
+    b.close()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("workload", choices=["c3", "c4"])
+    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--slots", type=int, default=192)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    a = ap.parse_args(argv)
+    lib = abi.load_library()
+    prec = abi.SCFT_FP32_SPLIT if a.precision == "fp32" else abi.SCFT_FP64
+    if a.workload == "c3":
+        run(lib, "C3 128x128x64", c3_params(), prec, a.iters)
+    else:
+        run(lib, "C4 sweep (NX 192/256/320)", c4_params(a.slots), prec, a.iters)
+
+
+if __name__ == "__main__":
+    main()
