@@ -1105,6 +1105,208 @@ __global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restri
     }
 }
 
+// -------------------------- register FFT, lengths 16 B (B = 12, 20: 192, 320)
+//
+// Periodic lines of length L = 16 B with B not a power of two: 16 lanes per
+// complex line, lane p holding x[p + 16 m] for m < B.  A B-point DFT per
+// lane (radix 4 x B/4), the twiddle w_L^{p k2}, one LDS transpose, then per
+// column k2 a 16-point DFT over p, K at spectrum index k2 + B k1, and the
+// inverse 16-point DFT; the transpose back, the inverse twiddle and the
+// inverse B-point DFT.  Column k2 = q + 16 r goes to lane q: with B = 20
+// lanes q < 4 take a second column.
+
+// w_N^m = exp(SIGN 2 pi i m / N), N = 12 or 20
+template <int N, int SIGN>
+__device__ __forceinline__ cpx<double> wN(int m)
+{
+    if constexpr (N == 12) {
+        constexpr double a = 0.8660254037844386467637;
+        const double C[12] = {1.0, a, 0.5, 0.0, -0.5, -a, -1.0, -a, -0.5, 0.0, 0.5, a};
+        const double S[12] = {0.0, 0.5, a, 1.0, a, 0.5, 0.0, -0.5, -a, -1.0, -a, -0.5};
+        return cpx<double>{C[m % 12], SIGN * S[m % 12]};
+    } else {
+        constexpr double a = 0.9510565162951535721164, b = 0.8090169943749474241023;
+        constexpr double c = 0.5877852522924731291687, d = 0.3090169943749474241023;
+        const double C[20] = {1.0, a, b, c, d, 0.0, -d, -c, -b, -a, -1.0, -a, -b, -c, -d, 0.0, d, c, b, a};
+        const double S[20] = {0.0, d, c, b, a, 1.0, a, b, c, d, 0.0, -d, -c, -b, -a, -1.0, -a, -b, -c, -d};
+        return cpx<double>{C[m % 20], SIGN * S[m % 20]};
+    }
+}
+
+// In-place N-point DFT of v[0 .. N-1], N = 4 N2 (N2 = 3, 5): n = n1 + 4 n2
+// (DFTs of N2 over n2), twiddle w_N^{n1 k2}, DFTs of 4 over n1; output
+// index k = k2 + N2 k1.
+template <int N, int SIGN>
+__device__ __forceinline__ void dft_4x(cpx<double> (&v)[N])
+{
+    constexpr int N2 = N / 4;
+    cpx<double> t[4][N2];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        cpx<double> a[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) a[n2] = v[n1 + 4 * n2];
+        dft_small<N2>(a, double(SIGN));
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) t[n1][k2] = (n1 * k2 == 0) ? a[k2] : cmul(a[k2], wN<N, SIGN>(n1 * k2));
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+        cpx<double> b[4] = {t[0][k2], t[1][k2], t[2][k2], t[3][k2]};
+        dft_small<4>(b, double(SIGN));
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) v[k2 + N2 * k1] = b[k1];
+    }
+}
+
+template <int B>
+struct FReg2 {
+    static constexpr int L = 16 * B;
+    static constexpr int G = 4;                 // complex lines per wave (16 lanes each)
+    static constexpr int RS = 17;               // tile row stride (complex)
+    static constexpr int RG = B * RS + 4;       // line tiles 4 slots apart in the bank rows
+    static constexpr int UNIT = 2 * G;          // real lines per wave unit
+    static constexpr int R = (B + 15) / 16;     // columns per lane
+    static constexpr int W = B > 16 ? 3 : 4;    // waves per workgroup: two workgroups per CU in LDS
+};
+
+template <int B, typename Tin>
+struct FReg2Loads {
+    Pair<Tin> x[B];
+};
+
+template <int B, typename Tin>
+__device__ __forceinline__ void freg2_fetch(const LineOp *__restrict__ ops, int upo, int u, int nunits, int g, int p,
+                                            FReg2Loads<B, Tin> &R)
+{
+    u = min(u, nunits - 1);
+    const LineOp &op = ops[u / upo];
+    const int r0 = (u % upo) * FReg2<B>::UNIT + 2 * g;
+    const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
+    const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
+    const Tin *src = static_cast<const Tin *>(op.src);
+    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = 16u * unsigned(op.stride);
+    if (op.pairs) {
+#pragma unroll
+        for (int m = 0; m < B; ++m) R.x[m] = ldg_pair(src, o0 + m * step);
+    } else {
+        const unsigned o1 = o0 + unsigned(b1 - b0);
+#pragma unroll
+        for (int m = 0; m < B; ++m) R.x[m] = Pair<Tin>{ldg(src, o0 + m * step), ldg(src, o1 + m * step)};
+    }
+}
+
+// The twiddles w_L^{-e} of an op's length-L table into LDS at e + e/16.
+template <int L>
+__device__ __forceinline__ void stage_twiddles(const LineOp &op, cpx<double> *TW, int tid, int nthreads)
+{
+    const double *twd = reinterpret_cast<const double *>(op.tw);
+    for (int e = tid; e < L; e += nthreads) TW[e + (e >> 4)] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
+}
+
+template <int B, typename Tout>
+__device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double> (&v)[B], cpx<double> *Tg,
+                                            const cpx<double> *TW, int p)
+{
+    using F = FReg2<B>;
+    constexpr int L = F::L, RS = F::RS, R = F::R;
+    const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
+    const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
+    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = 16u * unsigned(op.stride);
+    const unsigned o1 = o0 + unsigned(b1 - b0);
+    if (op.epre) {   // first axis of a step
+#pragma unroll
+        for (int m = 0; m < B; ++m) {
+            v[m].x *= pre_factor(op.epre, op.epre2, o0 + m * step);
+            v[m].y *= pre_factor(op.epre, op.epre2, o1 + m * step);
+        }
+    }
+    dft_4x<B, -1>(v);   // over m: v[k2]
+#pragma unroll
+    for (int k = 1; k < B; ++k) {
+        const int e = p * k;   // < L
+        v[k] = cmul(v[k], TW[e + (e >> 4)]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < B; ++k) Tg[k * RS + p] = v[k];
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int c = p + 16 * r;   // column k2 of this lane
+        if (c < B) {
+            cpx<double> col[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) col[k] = Tg[c * RS + k];
+            dft16<-1>(col);   // over p: spectrum index c + B k1
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) {
+                const double kk = ldg(op.K, unsigned(c + B * k1)) * (1.0 / L);   // 1/L folded in
+                col[k1].x *= kk;
+                col[k1].y *= kk;
+            }
+            dft16<+1>(col);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) Tg[c * RS + k] = col[k];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < B; ++k) v[k] = Tg[k * RS + p];
+#pragma unroll
+    for (int k = 1; k < B; ++k) {
+        const int e = p * k;
+        cpx<double> w = TW[e + (e >> 4)];
+        w.y = -w.y;
+        v[k] = cmul(v[k], w);
+    }
+    dft_4x<B, +1>(v);   // v[m] = x[p + 16 m] after the operator
+    Tout *dst = static_cast<Tout *>(op.dst);
+#pragma unroll
+    for (int m = 0; m < B; ++m) {
+        double y0 = v[m].x, y1 = v[m].y;
+        if (op.epost) {
+            y0 *= ldg(op.epost, o0 + m * step);
+            y1 *= ldg(op.epost, o1 + m * step);
+        }
+        if (op.pairs) {
+            if (ok0) stg_pair(dst, o0 + m * step, Tout(y0), Tout(y1));
+        } else {
+            if (ok0) stg(dst, o0 + m * step, Tout(y0));
+            if (ok1) stg(dst, o1 + m * step, Tout(y1));
+        }
+    }
+}
+
+template <int B, typename Tin, typename Tout>
+__global__ void __launch_bounds__(64 * FReg2<B>::W) k_fftreg2(const LineOp *__restrict__ ops, int nunits, int upo)
+{
+    using F = FReg2<B>;
+    constexpr int L = F::L, G = F::G, RG = F::RG, W = F::W;
+    __shared__ cpx<double> T[W * G * RG];
+    __shared__ cpx<double> TW[L + L / 16];
+    const int tid = threadIdx.x, lane = tid & 63;
+    stage_twiddles<L>(ops[0], TW, tid, 64 * W);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane % G, p = lane / G;   // line-minor
+    cpx<double> *Tg = T + (wave * G + g) * RG;
+    const int ustride = gridDim.x * W;
+    int u = blockIdx.x * W + wave;
+    if (u >= nunits) return;   // no workgroup barrier follows
+    FReg2Loads<B, Tin> ld;
+    freg2_fetch<B, Tin>(ops, upo, u, nunits, g, p, ld);
+    for (; u < nunits; u += ustride) {
+        const LineOp &op = ops[u / upo];
+        const int r0 = (u % upo) * F::UNIT + 2 * g;
+        cpx<double> v[B];
+#pragma unroll
+        for (int m = 0; m < B; ++m) v[m] = cpx<double>{double(ld.x[m].x), double(ld.x[m].y)};
+        freg2_fetch<B, Tin>(ops, upo, u + ustride, nunits, g, p, ld);   // in flight during the transforms
+        freg2_apply<B, Tout>(op, r0, v, Tg, TW, p);
+    }
+}
+
 // ------------------------------------------------------------ noise stream
 //
 // The reference's deviates (funGaussDev, SCFT.c:1769-1782): POSIX drand48,

@@ -205,8 +205,10 @@ struct Slot {
     std::vector<QSlot> qslots;
 };
 
-enum LaunchKind { LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFTREG = 5 };
-constexpr int kLaunchKinds = 6;
+enum LaunchKind {
+    LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFTREG = 5, LK_FFTREG2 = 6
+};
+constexpr int kLaunchKinds = 7;
 
 struct Launch {
     int kind;        // LaunchKind
@@ -809,6 +811,13 @@ int build_schedule(scft_ctx *c)
                     ln.P = (maxl + unit - 1) / unit;   // units per op
                     const int units = ln.count * ln.P;
                     ln.gx = std::max(1, std::min((units + kFRWaves - 1) / kFRWaves, 2 * c->ncu));
+                } else if (c->fft_reg && !dirichlet && (L == 192 || L == 320)) {
+                    // the same over 16 lanes of 12 or 20 points; two workgroups per CU
+                    ln.kind = LK_FFTREG2;
+                    const int unit = FReg2<12>::UNIT, wpg = (L == 320) ? FReg2<20>::W : FReg2<12>::W;
+                    ln.P = (maxl + unit - 1) / unit;
+                    const int units = ln.count * ln.P;
+                    ln.gx = std::max(1, std::min((units + wpg - 1) / wpg, 2 * c->ncu));
                 } else if (c->fft_wave && !dirichlet && fftwave_supported(L)) {
                     ln.kind = LK_FFTWAVE;
                     ln.gx = (maxl + 2 * kFftWaves - 1) / (2 * kFftWaves);
@@ -954,6 +963,14 @@ void launch_one(scft_ctx *c, const Launch &ln)
         break;
     case LK_FFTWAVE:
         launch_fftwave<Tin, Tout>(c, ln);
+        break;
+    case LK_FFTREG2:
+        if (ln.L == 320)
+            hipLaunchKernelGGL((k_fftreg2<20, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<20>::W), 0, st,
+                               c->lineops + ln.first, ln.count * ln.P, ln.P);
+        else
+            hipLaunchKernelGGL((k_fftreg2<12, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<12>::W), 0, st,
+                               c->lineops + ln.first, ln.count * ln.P, ln.P);
         break;
     case LK_FFTREG:
         if (ln.L == 256)
@@ -1697,7 +1714,8 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     const double ts = double(c->tsize);
     for (const Launch &ln : c->launches) {
         const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
-                          ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFTREG) && ln.pos == 0);
+                          ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFTREG ||
+                            ln.kind == LK_FFTREG2) && ln.pos == 0);
         if ((kind == 0) != is_z) continue;
         if (kind == 1 && ln.pos != 2 && ln.pos != 3) continue;
         sel.push_back(ln);
@@ -1765,9 +1783,10 @@ int scft_get_schedule(const scft_ctx *c, char *buf, int len)
     for (auto &ln : c->launches)
         if (ln.kind >= 0 && ln.kind < kLaunchKinds) n[ln.kind]++;
     std::snprintf(buf, len,
-                  "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fftreg=%d dense_mfma=%d dense_mfma2=%d graph=%d",
+                  "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fftreg=%d dense_mfma=%d dense_mfma2=%d graph=%d "
+                  "fftreg2=%d",
                   c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFTREG],
-                  n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0);
+                  n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0, n[LK_FFTREG2]);
     return SCFT_OK;
 }
 

@@ -154,11 +154,13 @@ def test_z_operator_sizes_match_oracle(abi, configs, orc, lib, zkind, nz, mask):
 @pytest.mark.parametrize("nx,ny,nz", [(16, 1, 6), (48, 1, 6), (80, 1, 6), (96, 1, 6), (160, 1, 6), (192, 1, 6),
                                       (320, 1, 6), (384, 1, 6), (512, 1, 6), (12, 1, 6), (32, 16, 6), (16, 48, 6),
                                       (64, 64, 6), (256, 1, 6), (256, 8, 6), (32, 256, 6), (256, 3, 5),
-                                      (8, 256, 5), (128, 1, 6), (128, 6, 5), (16, 128, 6), (128, 128, 5)])
+                                      (8, 256, 5), (128, 1, 6), (128, 6, 5), (16, 128, 6), (128, 128, 5),
+                                      (192, 3, 5), (8, 192, 6), (320, 3, 5), (6, 320, 6)])
 def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, ny, nz):
-    """x and y axes through the four-step register FFT (lengths 128, 256), the
-    wave-per-line FFT (compile-time lengths) and the generic LDS FFT, against
-    the oracle, on a thin film; odd NZ makes the strided lines odd-aligned."""
+    """x and y axes through the four-step register FFTs (lengths 128, 256 and
+    192, 320), the wave-per-line FFT (compile-time lengths) and the generic
+    LDS FFT, against the oracle, on a thin film; odd NZ makes the strided
+    lines odd-aligned."""
     os.environ["SCFT_XFFT"] = variant
     try:
         p = configs.chem_droplet(lx=nx * 0.25, ly=(ny * 0.25 if ny > 1 else 1.0), lz=nz * 0.25, dr=0.25, ns=11,
