@@ -19,6 +19,12 @@ The iterations-to-converge distribution is measured by running solves of
 this workload to convergence (profiles/c2_convergence_<precision>.json,
 produced by `python bench.py --converge N --precision P`).
 
+`roofline` is for the dominant kernel (the z operator): 2 b bytes per
+voxel-step over its launch time.  `iteration_roofline` is for the whole
+iteration: 2 b V S + 2 b V P + 8 V (3 n + 2) bytes per instance-iteration
+(S propagator steps, P Simpson points, n active fields) times the measured
+instance-iterations per second.
+
 Prints one JSON line.
 """
 import argparse
@@ -92,6 +98,27 @@ def conv_stats(prec_name):
     return {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)), "p99": float(np.percentile(it, 99)),
             "n": int(len(it)), "n_censored": len(cens), "censored_at": (min(cens) if cens else None),
             "source": "profiles/c2_convergence_%s.json" % prec_name}
+
+
+def iteration_bytes(abi, lib, params, b_el):
+    """Algorithmic bytes of one instance-iteration, 2 b V S + 2 b V P +
+    8 V (3 n + 2): S propagator steps (diblocks and brushes two chains,
+    homopolymers one), P Simpson points (a diblock's two blocks share their
+    junction point, counted in both), n active fields."""
+    import ctypes as C
+    d = abi.ScftDerived()
+    abi.check(lib, lib.scft_derive(C.byref(params), C.byref(d)), "scft_derive")
+    V = d.nx * d.ny * d.nz
+    S = P = 0
+    for sp in range(abi.NSP):
+        ns = d.ns_sp[sp]
+        if ns <= 0:
+            continue
+        chains = 1 if 4 <= sp < 14 else 2
+        S += chains * (ns - 1)
+        P += ns + 1 if sp < 4 else ns
+    n = d.n_active
+    return 2.0 * b_el * V * S + 2.0 * b_el * V * P + 8.0 * V * (3 * n + 2), {"V": V, "S": S, "P": P, "n": n}
 
 
 def cpu_baseline(abi, configs, seconds):
@@ -263,6 +290,10 @@ def main():
     alg_bytes = 2.0 * b_el * units
     ach = alg_bytes / (zt["avg_us"] * 1e-6) / 1e9 if zt["avg_us"] > 0 else None
     kern_gbs = zt["bytes"] / (zt["avg_us"] * 1e-6) / 1e9 if zt["avg_us"] > 0 else None
+    # the whole iteration against the same peak: algorithmic bytes of every
+    # instance-iteration over the measured time per instance-iteration
+    it_bytes, it_terms = iteration_bytes(abi, lib, configs.params_c2(), b_el)
+    it_gbs = it_bytes * inst_iter_per_s / 1e9
     line = {
         "metric": "converged SCFT solves/hour (p50/p99 time-to-converge at ||dw||_rms < 1e-5)",
         "value": value,
@@ -294,6 +325,9 @@ def main():
                      "fp64_tflops": zt["flops"] / (zt["avg_us"] * 1e-6) / 1e12 if zt["avg_us"] > 0 else None,
                      "x_axis_kernel": {"launch_us": xt["avg_us"], "bytes": xt["bytes"],
                                        "GBs": xt["bytes"] / (xt["avg_us"] * 1e-6) / 1e9 if xt["avg_us"] > 0 else None}},
+        "iteration_roofline": {"bytes_per_instance_iteration": it_bytes, "terms": it_terms,
+                               "achieved": it_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": it_gbs / PEAK_HBM_GBS},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
