@@ -14,14 +14,18 @@
 //     FFT of length 2(NZ+1) = 2*3*43 (NZ = 128) whose prime factor 43 costs
 //     ~6x that.  The autotuner picks per size.
 //   - k_fftreg<P>: periodic lines of length 16 P (128, 256) as a four-step
-//     FFT in registers; k_fftwave: other compile-time lengths, one
+//     FFT in registers; k_fftreg2<B>: the same for 16 B (192, 320), 16
+//     lanes of B points; k_fftwave: other compile-time lengths, one
 //     wavefront per line; k_lineop: any length, LDS Stockham passes, and the
 //     reference's Dirichlet transform (odd extension to N = 2(NZ+1),
 //     SCFT.c:2551-2592) when the dense z operator is not used.
 // All transform arithmetic is FP64; propagator slices (and the scratch
 // between axes) are stored as T (double, or float for the FP32-split
 // mode).  Every launch covers every live chain of every active instance of
-// the batch.
+// the batch.  Around the propagator: k_prep (Boltzmann factors, seed
+// start), k_qsum / k_density (Q and the contour sums), k_update / k_final
+// (the field update, F, RMSDev) and k_rng_* (the reference's noise stream
+// drawn on the device).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -1322,10 +1326,10 @@ __global__ void __launch_bounds__(64 * FReg2<B>::W) k_fftreg2(const LineOp *__re
 // writes the deviates in draw order and the state after the pair of the
 // last one, and k_rng_commit makes that the next iteration's state.  Each
 // instance of a batch has its own stream: grid y runs over the instances
-// that draw (RngSlot).  The
-// acceptance test is evaluated with the host's IEEE operations (no
-// contraction into FMA), so the accepted pairs, hence the state, are the
-// host's bit for bit; the deviate itself goes through the device log.
+// that draw (RngSlot).  The acceptance test is evaluated with the host's
+// IEEE operations (no contraction into FMA), so the accepted pairs, hence
+// the state, are the host's bit for bit; the deviate itself goes through
+// the device log.
 
 constexpr int kRngChunk = 32;   // candidate pairs per thread
 constexpr unsigned long long kLcgA = 0x5DEECE66DULL, kLcgC = 0xBULL, kLcgMask = 0xFFFFFFFFFFFFULL;
