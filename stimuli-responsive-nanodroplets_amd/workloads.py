@@ -2,8 +2,12 @@
 bench: per batched iteration and per launch of the z operator and the
 last-axis transform (HIP events, scft_time_launches).
 
+    python workloads.py c1 [--iters N] [--precision fp32|fp64]
+        2D bulk AB diblock melt, 64x1x64, NS 101, MASK 0, one solve
     python workloads.py c3 [--iters N] [--precision fp32|fp64]
         3D thin-film droplet, 128x128x64, NS 201 (C3 chemistry), one solve
+    python workloads.py c5 [--slots B] [--iters N] [--precision ...]
+        3D served requests, 96x96x48 (C3 chemistry), B resident
     python workloads.py c4 [--slots B] [--iters N] [--precision ...]
         phase-diagram sweep batch: B slots of the C2 chemistry with
         chi_AB N over 15..36 and chi_AC N over 20..55 (8 x 8 grid), LX drawn
@@ -28,8 +32,16 @@ import abi  # noqa: E402
 import configs  # noqa: E402
 
 
+def c1_params():
+    return [configs.params_c1()]
+
+
 def c3_params():
     return [configs.params_c3()]
+
+
+def c5_params(slots):
+    return [configs.params_c5_3d() for _ in range(slots)]
 
 
 def c4_params(slots, seed=0):
@@ -71,15 +83,19 @@ def run(lib, name, protos, precision, iters):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=["c3", "c4"])
+    ap.add_argument("workload", choices=["c1", "c3", "c4", "c5"])
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--slots", type=int, default=192)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     a = ap.parse_args(argv)
     lib = abi.load_library()
     prec = abi.SCFT_FP32_SPLIT if a.precision == "fp32" else abi.SCFT_FP64
-    if a.workload == "c3":
+    if a.workload == "c1":
+        run(lib, "C1 64x1x64", c1_params(), prec, a.iters)
+    elif a.workload == "c3":
         run(lib, "C3 128x128x64", c3_params(), prec, a.iters)
+    elif a.workload == "c5":
+        run(lib, "C5 96x96x48", c5_params(a.slots), prec, a.iters)
     else:
         run(lib, "C4 sweep (NX 192/256/320)", c4_params(a.slots), prec, a.iters)
 
