@@ -1109,15 +1109,15 @@ __global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restri
     }
 }
 
-// -------------------------- register FFT, lengths 16 B (B = 12, 20: 192, 320)
+// --------------- register FFT, lengths P B (16 x 12, 16 x 20, 8 x 12: 192, 320, 96)
 //
-// Periodic lines of length L = 16 B with B not a power of two: 16 lanes per
-// complex line, lane p holding x[p + 16 m] for m < B.  A B-point DFT per
+// Periodic lines of length L = P B with B not a power of two: P lanes per
+// complex line, lane p holding x[p + P m] for m < B.  A B-point DFT per
 // lane (radix 4 x B/4), the twiddle w_L^{p k2}, one LDS transpose, then per
-// column k2 a 16-point DFT over p, K at spectrum index k2 + B k1, and the
-// inverse 16-point DFT; the transpose back, the inverse twiddle and the
-// inverse B-point DFT.  Column k2 = q + 16 r goes to lane q: with B = 20
-// lanes q < 4 take a second column.
+// column k2 a P-point DFT over p, K at spectrum index k2 + B k1, and the
+// inverse P-point DFT; the transpose back, the inverse twiddle and the
+// inverse B-point DFT.  Column k2 = q + P r goes to lane q: when B > P
+// lanes q < B - P take a second column.
 
 // w_N^m = exp(SIGN 2 pi i m / N), N = 12 or 20
 template <int N, int SIGN>
@@ -1163,14 +1163,14 @@ __device__ __forceinline__ void dft_4x(cpx<double> (&v)[N])
     }
 }
 
-template <int B>
+template <int B, int P = 16>
 struct FReg2 {
-    static constexpr int L = 16 * B;
-    static constexpr int G = 4;                 // complex lines per wave (16 lanes each)
-    static constexpr int RS = 17;               // tile row stride (complex)
-    static constexpr int RG = B * RS + 4;       // line tiles 4 slots apart in the bank rows
+    static constexpr int L = P * B;
+    static constexpr int G = 64 / P;            // complex lines per wave (P lanes each)
+    static constexpr int RS = P + 1;            // tile row stride (complex)
+    static constexpr int RG = B * RS + 16 / G;  // line tiles 16/G slots apart in the bank rows
     static constexpr int UNIT = 2 * G;          // real lines per wave unit
-    static constexpr int R = (B + 15) / 16;     // columns per lane
+    static constexpr int R = (B + P - 1) / P;   // columns per lane
     static constexpr int W = B > 16 ? 3 : 4;    // waves per workgroup: two workgroups per CU in LDS
 };
 
@@ -1179,17 +1179,17 @@ struct FReg2Loads {
     Pair<Tin> x[B];
 };
 
-template <int B, typename Tin>
+template <int B, int P, typename Tin>
 __device__ __forceinline__ void freg2_fetch(const LineOp *__restrict__ ops, int upo, int u, int nunits, int g, int p,
                                             FReg2Loads<B, Tin> &R)
 {
     u = min(u, nunits - 1);
     const LineOp &op = ops[u / upo];
-    const int r0 = (u % upo) * FReg2<B>::UNIT + 2 * g;
+    const int r0 = (u % upo) * FReg2<B, P>::UNIT + 2 * g;
     const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
     const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
     const Tin *src = static_cast<const Tin *>(op.src);
-    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = 16u * unsigned(op.stride);
+    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = unsigned(P) * unsigned(op.stride);
     if (op.pairs) {
 #pragma unroll
         for (int m = 0; m < B; ++m) R.x[m] = ldg_pair(src, o0 + m * step);
@@ -1208,15 +1208,15 @@ __device__ __forceinline__ void stage_twiddles(const LineOp &op, cpx<double> *TW
     for (int e = tid; e < L; e += nthreads) TW[e + (e >> 4)] = cpx<double>{ldg(twd, 2 * e), ldg(twd, 2 * e + 1)};
 }
 
-template <int B, typename Tout>
+template <int B, int P, typename Tout>
 __device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double> (&v)[B], cpx<double> *Tg,
                                             const cpx<double> *TW, int p)
 {
-    using F = FReg2<B>;
+    using F = FReg2<B, P>;
     constexpr int L = F::L, RS = F::RS, R = F::R;
     const bool ok0 = r0 < op.nlines, ok1 = r0 + 1 < op.nlines;
     const int b0 = line_base(op, ok0 ? r0 : 0), b1 = line_base(op, ok1 ? r0 + 1 : 0);
-    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = 16u * unsigned(op.stride);
+    const unsigned o0 = unsigned(b0) + unsigned(p) * unsigned(op.stride), step = unsigned(P) * unsigned(op.stride);
     const unsigned o1 = o0 + unsigned(b1 - b0);
     if (op.epre) {   // first axis of a step
 #pragma unroll
@@ -1237,21 +1237,21 @@ __device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double
     wave_sync();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int c = p + 16 * r;   // column k2 of this lane
+        const int c = p + P * r;   // column k2 of this lane
         if (c < B) {
-            cpx<double> col[16];
+            cpx<double> col[P];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) col[k] = Tg[c * RS + k];
-            dft16<-1>(col);   // over p: spectrum index c + B k1
+            for (int k = 0; k < P; ++k) col[k] = Tg[c * RS + k];
+            dft_p<P, -1>(col, 0);   // over p: spectrum index c + B k1
 #pragma unroll
-            for (int k1 = 0; k1 < 16; ++k1) {
+            for (int k1 = 0; k1 < P; ++k1) {
                 const double kk = ldg(op.K, unsigned(c + B * k1)) * (1.0 / L);   // 1/L folded in
                 col[k1].x *= kk;
                 col[k1].y *= kk;
             }
-            dft16<+1>(col);
+            dft_p<P, +1>(col, 0);
 #pragma unroll
-            for (int k = 0; k < 16; ++k) Tg[c * RS + k] = col[k];
+            for (int k = 0; k < P; ++k) Tg[c * RS + k] = col[k];
         }
     }
     wave_sync();
@@ -1264,7 +1264,7 @@ __device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double
         w.y = -w.y;
         v[k] = cmul(v[k], w);
     }
-    dft_4x<B, +1>(v);   // v[m] = x[p + 16 m] after the operator
+    dft_4x<B, +1>(v);   // v[m] = x[p + P m] after the operator
     Tout *dst = static_cast<Tout *>(op.dst);
 #pragma unroll
     for (int m = 0; m < B; ++m) {
@@ -1282,10 +1282,10 @@ __device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double
     }
 }
 
-template <int B, typename Tin, typename Tout>
-__global__ void __launch_bounds__(64 * FReg2<B>::W) k_fftreg2(const LineOp *__restrict__ ops, int nunits, int upo)
+template <int B, int P, typename Tin, typename Tout>
+__global__ void __launch_bounds__((64 * FReg2<B, P>::W)) k_fftreg2(const LineOp *__restrict__ ops, int nunits, int upo)
 {
-    using F = FReg2<B>;
+    using F = FReg2<B, P>;
     constexpr int L = F::L, G = F::G, RG = F::RG, W = F::W;
     __shared__ cpx<double> T[W * G * RG];
     __shared__ cpx<double> TW[L + L / 16];
@@ -1299,15 +1299,15 @@ __global__ void __launch_bounds__(64 * FReg2<B>::W) k_fftreg2(const LineOp *__re
     int u = blockIdx.x * W + wave;
     if (u >= nunits) return;   // no workgroup barrier follows
     FReg2Loads<B, Tin> ld;
-    freg2_fetch<B, Tin>(ops, upo, u, nunits, g, p, ld);
+    freg2_fetch<B, P, Tin>(ops, upo, u, nunits, g, p, ld);
     for (; u < nunits; u += ustride) {
         const LineOp &op = ops[u / upo];
         const int r0 = (u % upo) * F::UNIT + 2 * g;
         cpx<double> v[B];
 #pragma unroll
         for (int m = 0; m < B; ++m) v[m] = cpx<double>{double(ld.x[m].x), double(ld.x[m].y)};
-        freg2_fetch<B, Tin>(ops, upo, u + ustride, nunits, g, p, ld);   // in flight during the transforms
-        freg2_apply<B, Tout>(op, r0, v, Tg, TW, p);
+        freg2_fetch<B, P, Tin>(ops, upo, u + ustride, nunits, g, p, ld);   // in flight during the transforms
+        freg2_apply<B, P, Tout>(op, r0, v, Tg, TW, p);
     }
 }
 

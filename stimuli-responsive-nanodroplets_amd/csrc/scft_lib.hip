@@ -811,10 +811,11 @@ int build_schedule(scft_ctx *c)
                     ln.P = (maxl + unit - 1) / unit;   // units per op
                     const int units = ln.count * ln.P;
                     ln.gx = std::max(1, std::min((units + kFRWaves - 1) / kFRWaves, 2 * c->ncu));
-                } else if (c->fft_reg && !dirichlet && (L == 192 || L == 320)) {
-                    // the same over 16 lanes of 12 or 20 points; two workgroups per CU
+                } else if (c->fft_reg && !dirichlet && (L == 192 || L == 320 || L == 96)) {
+                    // the same over 16 lanes of 12 or 20 points, or 8 lanes of 12 (96)
                     ln.kind = LK_FFTREG2;
-                    const int unit = FReg2<12>::UNIT, wpg = (L == 320) ? FReg2<20>::W : FReg2<12>::W;
+                    const int unit = (L == 96) ? FReg2<12, 8>::UNIT : FReg2<12>::UNIT;
+                    const int wpg = (L == 320) ? FReg2<20>::W : FReg2<12>::W;
                     ln.P = (maxl + unit - 1) / unit;
                     const int units = ln.count * ln.P;
                     ln.gx = std::max(1, std::min((units + wpg - 1) / wpg, 2 * c->ncu));
@@ -966,10 +967,13 @@ void launch_one(scft_ctx *c, const Launch &ln)
         break;
     case LK_FFTREG2:
         if (ln.L == 320)
-            hipLaunchKernelGGL((k_fftreg2<20, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<20>::W), 0, st,
+            hipLaunchKernelGGL((k_fftreg2<20, 16, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<20>::W), 0, st,
+                               c->lineops + ln.first, ln.count * ln.P, ln.P);
+        else if (ln.L == 96)
+            hipLaunchKernelGGL((k_fftreg2<12, 8, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<12, 8>::W), 0, st,
                                c->lineops + ln.first, ln.count * ln.P, ln.P);
         else
-            hipLaunchKernelGGL((k_fftreg2<12, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<12>::W), 0, st,
+            hipLaunchKernelGGL((k_fftreg2<12, 16, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<12>::W), 0, st,
                                c->lineops + ln.first, ln.count * ln.P, ln.P);
         break;
     case LK_FFTREG:

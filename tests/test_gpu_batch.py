@@ -155,10 +155,11 @@ def test_z_operator_sizes_match_oracle(abi, configs, orc, lib, zkind, nz, mask):
                                       (320, 1, 6), (384, 1, 6), (512, 1, 6), (12, 1, 6), (32, 16, 6), (16, 48, 6),
                                       (64, 64, 6), (256, 1, 6), (256, 8, 6), (32, 256, 6), (256, 3, 5),
                                       (8, 256, 5), (128, 1, 6), (128, 6, 5), (16, 128, 6), (128, 128, 5),
-                                      (192, 3, 5), (8, 192, 6), (320, 3, 5), (6, 320, 6)])
+                                      (192, 3, 5), (8, 192, 6), (320, 3, 5), (6, 320, 6), (96, 3, 5),
+                                      (8, 96, 6), (96, 96, 5)])
 def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, ny, nz):
     """x and y axes through the four-step register FFTs (lengths 128, 256 and
-    192, 320), the wave-per-line FFT (compile-time lengths) and the generic
+    96, 192, 320), the wave-per-line FFT (compile-time lengths) and the generic
     LDS FFT, against the oracle, on a thin film; odd NZ makes the strided
     lines odd-aligned."""
     os.environ["SCFT_XFFT"] = variant
@@ -181,6 +182,40 @@ def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, n
         wa, _ = ctx.get_fields()
         wb, _ = ref.get_fields()
         assert _rel(wa, wb) <= TOL
+        ctx.close()
+    finally:
+        del os.environ["SCFT_XFFT"]
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(96, 3, 5), (8, 96, 6), (96, 96, 5), (192, 3, 5), (320, 3, 5), (6, 320, 6),
+                                      (128, 6, 5), (256, 3, 5)])
+def test_periodic_register_ffts_fp32_split(abi, configs, orc, lib, nx, ny, nz):
+    """The register FFTs with FP32 propagator slices (FP64 arithmetic)
+    against the FP64 oracle: F within 1e-5 of max(1, |F|), the updated
+    fields within 1e-5 of their largest value, over two iterations."""
+    os.environ["SCFT_XFFT"] = "reg"
+    try:
+        p = configs.chem_droplet(lx=nx * 0.25, ly=(ny * 0.25 if ny > 1 else 1.0), lz=nz * 0.25, dr=0.25, ns=11,
+                                 vol_dbk=0.2, vol_s1=0.05, len_s1=0.3)
+        ctx = abi.Context(lib, p, abi.SCFT_FP32_SPLIT)
+        sched = ctx.schedule()
+        ref = orc.state(p)
+        d = ctx.derived
+        act = [abi.FIELD_NAMES[f] for f in range(13) if d.active[f]]
+        w, pf = configs.random_fields(p, seed=6, amp=0.5, fields=act)
+        for s in (ctx, ref):
+            s.set_fields(w, pf)
+            s.set_mask(configs.brush_mask(p))
+        for it in (2, 3):
+            a, o = ctx.iterate(it), ref.iterate(it)
+            print((nx, ny, nz), it, "dF = %.3e" % abs(a.free_energy - o.free_energy), sched, flush=True)
+            assert _srel(a.free_energy, o.free_energy) <= 1e-5, (a.free_energy, o.free_energy, sched)
+        wa, _ = ctx.get_fields()
+        wb, _ = ref.get_fields()
+        assert _rel(wa, wb) <= 1e-5
+        n = dict(kv.split("=") for kv in ctx.schedule().split() if "=" in kv and kv[0] != "z")
+        kind = "fftreg2" if {nx, ny} & {96, 192, 320} else "fftreg"
+        assert int(n[kind]) > 0, ctx.schedule()
         ctx.close()
     finally:
         del os.environ["SCFT_XFFT"]
