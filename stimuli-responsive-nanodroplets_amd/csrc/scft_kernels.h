@@ -1109,11 +1109,11 @@ __global__ void __launch_bounds__(64 * kFRWaves) k_fftreg(const LineOp *__restri
     }
 }
 
-// --------------- register FFT, lengths P B (16 x 12, 16 x 20, 8 x 12: 192, 320, 96)
+// ------- register FFT, lengths P B (16 x 12, 16 x 20, 8 x 12, 8 x 8: 192, 320, 96, 64)
 //
-// Periodic lines of length L = P B with B not a power of two: P lanes per
-// complex line, lane p holding x[p + P m] for m < B.  A B-point DFT per
-// lane (radix 4 x B/4), the twiddle w_L^{p k2}, one LDS transpose, then per
+// Periodic lines of length L = P B: P lanes per complex line, lane p
+// holding x[p + P m] for m < B.  A B-point DFT per lane (radix 4 x B/4, or
+// the 8-point form), the twiddle w_L^{p k2}, one LDS transpose, then per
 // column k2 a P-point DFT over p, K at spectrum index k2 + B k1, and the
 // inverse P-point DFT; the transpose back, the inverse twiddle and the
 // inverse B-point DFT.  Column k2 = q + P r goes to lane q: when B > P
@@ -1163,6 +1163,14 @@ __device__ __forceinline__ void dft_4x(cpx<double> (&v)[N])
     }
 }
 
+// The B-point DFT of one lane: radix 4 x B/4, or the 8-point form.
+template <int B, int SIGN>
+__device__ __forceinline__ void dft_lane(cpx<double> (&v)[B])
+{
+    if constexpr (B == 8) dft8<SIGN>(v, 0);
+    else dft_4x<B, SIGN>(v);
+}
+
 template <int B, int P = 16>
 struct FReg2 {
     static constexpr int L = P * B;
@@ -1172,6 +1180,7 @@ struct FReg2 {
     static constexpr int UNIT = 2 * G;          // real lines per wave unit
     static constexpr int R = (B + P - 1) / P;   // columns per lane
     static constexpr int W = B > 16 ? 3 : 4;    // waves per workgroup: two workgroups per CU in LDS
+    static constexpr int WG_PER_CU = L <= 64 ? 4 : 2;   // workgroups that fit one CU's LDS
 };
 
 template <int B, typename Tin>
@@ -1225,7 +1234,7 @@ __device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double
             v[m].y *= pre_factor(op.epre, op.epre2, o1 + m * step);
         }
     }
-    dft_4x<B, -1>(v);   // over m: v[k2]
+    dft_lane<B, -1>(v);   // over m: v[k2]
 #pragma unroll
     for (int k = 1; k < B; ++k) {
         const int e = p * k;   // < L
@@ -1264,7 +1273,7 @@ __device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double
         w.y = -w.y;
         v[k] = cmul(v[k], w);
     }
-    dft_4x<B, +1>(v);   // v[m] = x[p + P m] after the operator
+    dft_lane<B, +1>(v);   // v[m] = x[p + P m] after the operator
     Tout *dst = static_cast<Tout *>(op.dst);
 #pragma unroll
     for (int m = 0; m < B; ++m) {

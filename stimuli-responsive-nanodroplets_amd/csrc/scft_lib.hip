@@ -157,9 +157,14 @@ struct DenseTable {
 
 enum ZKind { Z_FFT = 0, Z_DENSE = 1 };
 
-// Process-wide autotuner results: (n, dirichlet, alpha bits, storage) -> ZKind.
+// Process-wide autotuner results: (n, dirichlet, alpha bits, storage,
+// slots timed) -> ZKind.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, long long, int>, int> g_tune;
+std::map<std::tuple<int, int, long long, int, int>, int> g_tune;
+
+// The autotuner times as many slots as put this many voxels in one launch
+// (at most the batch): one small slot alone times launch latency only.
+constexpr long long kTuneVoxels = 1LL << 22;
 
 }  // namespace
 
@@ -242,7 +247,8 @@ struct scft_ctx {
     std::vector<std::unique_ptr<DenseTable>> denses;
     int zkind = Z_FFT;
     bool fft_wave = true;            // wave-per-line FFT for supported periodic lengths
-    bool fft_reg = true;             // four-step register FFT for periodic lengths 128 and 256
+    bool fft_reg = true;             // register FFTs for periodic lengths 64, 96, 128, 192, 256, 320
+    bool fft_reg_force = false;      // register FFTs also for launches of few lines
     bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
     bool dense_mfma1 = false;        // force the one-wave-per-SIMD MFMA kernel for even NZ too
     bool tmp_wide = false;           // FP64 scratch between axis operators in FP32 mode
@@ -804,21 +810,27 @@ int build_schedule(scft_ctx *c)
                     }
                 ln.count = int(lops.size()) - ln.first;
                 ln.L = L;
-                if (c->fft_reg && !dirichlet && (L == 128 || L == 256)) {
+                const bool reg1 = c->fft_reg && !dirichlet && (L == 128 || L == 256);
+                const bool reg2 = c->fft_reg && !dirichlet && (L == 192 || L == 320 || L == 96 || L == 64);
+                const int unit = reg1 ? (L == 256 ? FReg<16>::UNIT : FReg<8>::UNIT)
+                                      : (L <= 96 ? FReg2<12, 8>::UNIT : FReg2<12>::UNIT);
+                const int units = ln.count * ((maxl + unit - 1) / unit);
+                // A register-FFT launch of fewer units (one wave each) than two
+                // per CU leaves most SIMDs idle for the whole transform; the
+                // wave-per-line FFT gives every line pair its own wave there.
+                const bool few = !c->fft_reg_force && c->fft_wave && fftwave_supported(L) && units < 2 * c->ncu;
+                if (reg1 && !few) {
                     // persistent waves over (op, unit of 2G real lines); two workgroups per CU
                     ln.kind = LK_FFTREG;
-                    const int unit = (L == 256) ? FReg<16>::UNIT : FReg<8>::UNIT;
                     ln.P = (maxl + unit - 1) / unit;   // units per op
-                    const int units = ln.count * ln.P;
                     ln.gx = std::max(1, std::min((units + kFRWaves - 1) / kFRWaves, 2 * c->ncu));
-                } else if (c->fft_reg && !dirichlet && (L == 192 || L == 320 || L == 96)) {
-                    // the same over 16 lanes of 12 or 20 points, or 8 lanes of 12 (96)
+                } else if (reg2 && !few) {
+                    // the same over 16 lanes of 12 or 20 points, 8 lanes of 12 (96) or of 8 (64)
                     ln.kind = LK_FFTREG2;
-                    const int unit = (L == 96) ? FReg2<12, 8>::UNIT : FReg2<12>::UNIT;
                     const int wpg = (L == 320) ? FReg2<20>::W : FReg2<12>::W;
+                    const int wgcu = (L == 64) ? FReg2<8, 8>::WG_PER_CU : FReg2<12>::WG_PER_CU;
                     ln.P = (maxl + unit - 1) / unit;
-                    const int units = ln.count * ln.P;
-                    ln.gx = std::max(1, std::min((units + wpg - 1) / wpg, 2 * c->ncu));
+                    ln.gx = std::max(1, std::min((units + wpg - 1) / wpg, wgcu * c->ncu));
                 } else if (c->fft_wave && !dirichlet && fftwave_supported(L)) {
                     ln.kind = LK_FFTWAVE;
                     ln.gx = (maxl + 2 * kFftWaves - 1) / (2 * kFftWaves);
@@ -971,6 +983,9 @@ void launch_one(scft_ctx *c, const Launch &ln)
                                c->lineops + ln.first, ln.count * ln.P, ln.P);
         else if (ln.L == 96)
             hipLaunchKernelGGL((k_fftreg2<12, 8, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<12, 8>::W), 0, st,
+                               c->lineops + ln.first, ln.count * ln.P, ln.P);
+        else if (ln.L == 64)
+            hipLaunchKernelGGL((k_fftreg2<8, 8, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<8, 8>::W), 0, st,
                                c->lineops + ln.first, ln.count * ln.P, ln.P);
         else
             hipLaunchKernelGGL((k_fftreg2<12, 16, Tin, Tout>), dim3(ln.gx), dim3(64 * FReg2<12>::W), 0, st,
@@ -1129,8 +1144,9 @@ void free_ctx(scft_ctx *c)
     delete c;
 }
 
-// Autotune the z operator for this context's size: time one step of every
-// chain of one slot with each candidate and keep the faster.
+// Autotune the z operator for this context's size: time the z launch of
+// the first step over every chain of a batch-sized set of slots with each
+// candidate and keep the faster.
 int autotune_z(scft_ctx *c)
 {
     const scft_params &p = c->proto;
@@ -1139,7 +1155,11 @@ int autotune_z(scft_ctx *c)
     const double az = DS / (DZ * DZ);
     long long abits;
     std::memcpy(&abits, &az, sizeof(abits));
-    auto key = std::make_tuple(c->NZ, p.mask ? 1 : 0, abits, int(c->tsize));
+    scft_derived d0;
+    if (scft_derive(&p, &d0) != SCFT_OK) return SCFT_EINVAL;
+    const long long v0 = std::max(1LL, (long long)d0.nx * d0.ny * d0.nz);
+    const int nt = int(std::min<long long>(c->nslots, std::max(1LL, (kTuneVoxels + v0 - 1) / v0)));
+    auto key = std::make_tuple(c->NZ, p.mask ? 1 : 0, abits, int(c->tsize), nt);
     const int hs = c->NZ - c->NZ / 2;
     const bool dense_ok = hs <= kDenseHMax;
     // SCFT_ZKIND=fft|dense forces the z operator (tests use it to check both)
@@ -1162,8 +1182,10 @@ int autotune_z(scft_ctx *c)
         double best = 1e30;
         for (int kind : {Z_FFT, Z_DENSE}) {
             c->zkind = kind;
-            RETCHK(slot_init(c, c->slots[0], c->proto));
-            c->slots[0].active = true;
+            for (int t = 0; t < nt; ++t) {
+                RETCHK(slot_init(c, c->slots[t], c->proto));
+                c->slots[t].active = true;
+            }
             RETCHK(build_schedule(c));
             // time the z launches of step 0 only
             hipEvent_t e0, e1;
@@ -1188,8 +1210,10 @@ int autotune_z(scft_ctx *c)
             (void)hipEventDestroy(e1);
             if (ms < best) { best = ms; choice = kind; }
         }
-        c->slots[0].active = false;
-        c->slots[0].ready = false;
+        for (int t = 0; t < nt; ++t) {
+            c->slots[t].active = false;
+            c->slots[t].ready = false;
+        }
         c->dirty = true;
     }
     c->zkind = choice;
@@ -1247,11 +1271,13 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     const char *g = std::getenv("SCFT_GRAPH");
     c->use_graph = !(g && g[0] == '0');
     // kernel variants (tests force each one): SCFT_XFFT=lds selects the
-    // generic LDS FFT for periodic axes, SCFT_DENSE=valu the VALU dense z op,
+    // generic LDS FFT for periodic axes, =wave the wave-per-line FFT, =reg
+    // the register FFTs at every size; SCFT_DENSE=valu the VALU dense z op,
     // SCFT_TMP=fp64 FP64 scratch between axis operators in FP32 mode
     const char *xf = std::getenv("SCFT_XFFT");
     c->fft_wave = !(xf && std::strcmp(xf, "lds") == 0);
     c->fft_reg = !(xf && (std::strcmp(xf, "lds") == 0 || std::strcmp(xf, "wave") == 0));
+    c->fft_reg_force = xf && std::strcmp(xf, "reg") == 0;
     const char *dm = std::getenv("SCFT_DENSE");
     c->dense_mfma = !(dm && std::strcmp(dm, "valu") == 0);
     c->dense_mfma1 = dm && std::strcmp(dm, "mfma1") == 0;

@@ -2,17 +2,17 @@
 bench: per batched iteration and per launch of the z operator and the
 last-axis transform (HIP events, scft_time_launches).
 
-    python workloads.py c1 [--iters N] [--precision fp32|fp64]
-        2D bulk AB diblock melt, 64x1x64, NS 101, MASK 0, one solve
-    python workloads.py c3 [--iters N] [--precision fp32|fp64]
-        3D thin-film droplet, 128x128x64, NS 201 (C3 chemistry), one solve
+    python workloads.py c1 [--slots B] [--iters N] [--precision fp32|fp64]
+        2D bulk AB diblock melt, 64x1x64, NS 101, MASK 0; B resident (1)
+    python workloads.py c3 [--slots B] [--iters N] [--precision fp32|fp64]
+        3D thin-film droplet, 128x128x64, NS 201 (C3 chemistry); B resident (1)
     python workloads.py c5 [--slots B] [--iters N] [--precision ...]
-        3D served requests, 96x96x48 (C3 chemistry), B resident
+        3D served requests, 96x96x48 (C3 chemistry), B resident (8)
     python workloads.py c4 [--slots B] [--iters N] [--precision ...]
         phase-diagram sweep batch: B slots of the C2 chemistry with
         chi_AB N over 15..36 and chi_AC N over 20..55 (8 x 8 grid), LX drawn
         per slot from {24, 32, 40} R_g (seeded), so grids are 192/256/320 x
-        1 x 128 of uneven size in one batch
+        1 x 128 of uneven size in one batch; B resident (192)
 
 Prints one line per workload.  Fields start from the droplet seed; the
 timed iterations follow two warm-up iterations.
@@ -32,12 +32,12 @@ import abi  # noqa: E402
 import configs  # noqa: E402
 
 
-def c1_params():
-    return [configs.params_c1()]
+def c1_params(slots):
+    return [configs.params_c1() for _ in range(slots)]
 
 
-def c3_params():
-    return [configs.params_c3()]
+def c3_params(slots):
+    return [configs.params_c3() for _ in range(slots)]
 
 
 def c5_params(slots):
@@ -85,19 +85,20 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("workload", choices=["c1", "c3", "c4", "c5"])
     ap.add_argument("--iters", type=int, default=10)
-    ap.add_argument("--slots", type=int, default=192)
+    ap.add_argument("--slots", type=int, default=None)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     a = ap.parse_args(argv)
     lib = abi.load_library()
     prec = abi.SCFT_FP32_SPLIT if a.precision == "fp32" else abi.SCFT_FP64
+    slots = a.slots or {"c1": 1, "c3": 1, "c4": 192, "c5": 8}[a.workload]
     if a.workload == "c1":
-        run(lib, "C1 64x1x64", c1_params(), prec, a.iters)
+        run(lib, "C1 64x1x64", c1_params(slots), prec, a.iters)
     elif a.workload == "c3":
-        run(lib, "C3 128x128x64", c3_params(), prec, a.iters)
+        run(lib, "C3 128x128x64", c3_params(slots), prec, a.iters)
     elif a.workload == "c5":
-        run(lib, "C5 96x96x48", c5_params(a.slots), prec, a.iters)
+        run(lib, "C5 96x96x48", c5_params(slots), prec, a.iters)
     else:
-        run(lib, "C4 sweep (NX 192/256/320)", c4_params(a.slots), prec, a.iters)
+        run(lib, "C4 sweep (NX 192/256/320)", c4_params(slots), prec, a.iters)
 
 
 if __name__ == "__main__":

@@ -156,10 +156,10 @@ def test_z_operator_sizes_match_oracle(abi, configs, orc, lib, zkind, nz, mask):
                                       (64, 64, 6), (256, 1, 6), (256, 8, 6), (32, 256, 6), (256, 3, 5),
                                       (8, 256, 5), (128, 1, 6), (128, 6, 5), (16, 128, 6), (128, 128, 5),
                                       (192, 3, 5), (8, 192, 6), (320, 3, 5), (6, 320, 6), (96, 3, 5),
-                                      (8, 96, 6), (96, 96, 5)])
+                                      (8, 96, 6), (96, 96, 5), (64, 3, 5), (8, 64, 6), (64, 1, 6)])
 def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, ny, nz):
     """x and y axes through the four-step register FFTs (lengths 128, 256 and
-    96, 192, 320), the wave-per-line FFT (compile-time lengths) and the generic
+    64, 96, 192, 320), the wave-per-line FFT (compile-time lengths) and the generic
     LDS FFT, against the oracle, on a thin film; odd NZ makes the strided
     lines odd-aligned."""
     os.environ["SCFT_XFFT"] = variant
@@ -188,7 +188,7 @@ def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, n
 
 
 @pytest.mark.parametrize("nx,ny,nz", [(96, 3, 5), (8, 96, 6), (96, 96, 5), (192, 3, 5), (320, 3, 5), (6, 320, 6),
-                                      (128, 6, 5), (256, 3, 5)])
+                                      (128, 6, 5), (256, 3, 5), (64, 3, 5), (64, 64, 5)])
 def test_periodic_register_ffts_fp32_split(abi, configs, orc, lib, nx, ny, nz):
     """The register FFTs with FP32 propagator slices (FP64 arithmetic)
     against the FP64 oracle: F within 1e-5 of max(1, |F|), the updated
@@ -214,7 +214,7 @@ def test_periodic_register_ffts_fp32_split(abi, configs, orc, lib, nx, ny, nz):
         wb, _ = ref.get_fields()
         assert _rel(wa, wb) <= 1e-5
         n = dict(kv.split("=") for kv in ctx.schedule().split() if "=" in kv and kv[0] != "z")
-        kind = "fftreg2" if {nx, ny} & {96, 192, 320} else "fftreg"
+        kind = "fftreg2" if {nx, ny} & {64, 96, 192, 320} else "fftreg"
         assert int(n[kind]) > 0, ctx.schedule()
         ctx.close()
     finally:
