@@ -157,7 +157,12 @@ int scft_iterate(scft_ctx *ctx, unsigned long iter, float mf,
  * completed iteration (what restart_random.dat holds).  These act on slot
  * 0; scft_slot_rng_seed / scft_slot_rng_state below give every slot of a
  * batch its own stream.  Replaces the host-side drand48 loop of
- * SCFT.c:2103.
+ * SCFT.c:2103.  The candidate pairs drawn per iteration are sized for the
+ * expected acceptance (pi/4) plus a margin of about 17 standard deviations;
+ * if an iteration still finds too few accepted pairs the call returns
+ * SCFT_ERNG and the generator state stays where it was, but that
+ * iteration's field update has already added an incomplete set of
+ * deviates: restore the fields from before the call before going on.
  */
 int scft_rng_seed(scft_ctx *ctx, const unsigned short xsubi[3]);
 int scft_rng_state(scft_ctx *ctx, unsigned short xsubi[3]);
