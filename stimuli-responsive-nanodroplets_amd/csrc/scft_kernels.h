@@ -1320,6 +1320,141 @@ __global__ void __launch_bounds__((64 * FReg2<B, P>::W)) k_fftreg2(const LineOp 
     }
 }
 
+// ------------------------------------------- fused step on a periodic 64 x 64 plane
+//
+// One contour step of one chain on a periodic 2D plane (NY = 1) of 64 x 64
+// points in one workgroup: the slice times its pre-factor into LDS
+// (coalesced), the 64-point z transforms with Kz over the rows (two real
+// rows per complex line), the 64-point x transforms with Kx over the
+// columns (two real columns per complex line), and the plane out
+// (coalesced).  Each complex line runs the 8 x 8 register FFT of k_fftreg2
+// with its transpose done in the line's own LDS cells.  It replaces the z
+// and x launches of a step: the scratch slice between the two axes stays
+// on chip.
+
+struct PlaneOp {
+    const void *src;           // slice t (T), x-major, z fastest
+    void *dst;                 // slice t + 1 (T)
+    const double *epre;        // E of this step
+    const double *epre2;       // factor the stored slice owes (null, epre, or another E)
+    const double *Kz, *Kx;     // 64-point spectra, natural order
+    const double *twz, *twx;   // w_64^{-e}, complex interleaved
+};
+
+constexpr int kPlaneN = 64;        // points per axis
+constexpr int kPlanePS = 66;       // LDS row stride (doubles): the rows of a half-wave start 8 banks apart
+constexpr int kPlaneWaves = 4;     // 32 complex lines per axis, 8 per wave
+
+// The operator on one complex line whose slot j (0..63) lives at Pl[a + j st]
+// (real part) and Pl[b + j st] (imaginary part).  Lane p holds slots p + 8 m
+// in v on entry and the operated values there on exit; the transpose uses
+// the line's own cells (each lane writes the cells it read).
+__device__ __forceinline__ void plane_line(cpx<double> (&v)[8], double *Pl, int a, int b, int st,
+                                           const cpx<double> *TW, const double *__restrict__ K, int p)
+{
+    dft8<-1>(v, 0);   // over m: v[k2]
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        const int e = p * k;
+        v[k] = cmul(v[k], TW[e + (e >> 4)]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int j = (k * 8 + p) * st;
+        Pl[a + j] = v[k].x;
+        Pl[b + j] = v[k].y;
+    }
+    wave_sync();
+    cpx<double> col[8];   // column k2 = p: spectrum index p + 8 k1
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int j = (p * 8 + k) * st;
+        col[k] = cpx<double>{Pl[a + j], Pl[b + j]};
+    }
+    dft8<-1>(col, 0);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) {
+        const double kk = ldg(K, unsigned(p + 8 * k1)) * (1.0 / 64);   // 1/L folded in (exact)
+        col[k1].x *= kk;
+        col[k1].y *= kk;
+    }
+    dft8<+1>(col, 0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int j = (p * 8 + k) * st;
+        Pl[a + j] = col[k].x;
+        Pl[b + j] = col[k].y;
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int j = (k * 8 + p) * st;
+        v[k] = cpx<double>{Pl[a + j], Pl[b + j]};
+    }
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        const int e = p * k;
+        cpx<double> w = TW[e + (e >> 4)];
+        w.y = -w.y;
+        v[k] = cmul(v[k], w);
+    }
+    dft8<+1>(v, 0);   // v[m] = slot p + 8 m after the operator
+}
+
+// grid: persistent workgroups over the ops of the launch
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(64 * kPlaneWaves) k_plane64(const PlaneOp *__restrict__ ops, int nops)
+{
+    constexpr int N = kPlaneN, PS = kPlanePS, NT = 64 * kPlaneWaves;
+    __shared__ double Pl[N * PS];
+    __shared__ cpx<double> TWz[N + N / 16], TWx[N + N / 16];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane & 7, p = lane >> 3;   // line-minor
+    for (int e = tid; e < N; e += NT) {      // every op of a launch shares the tables
+        TWz[e + (e >> 4)] = cpx<double>{ldg(ops[0].twz, 2 * e), ldg(ops[0].twz, 2 * e + 1)};
+        TWx[e + (e >> 4)] = cpx<double>{ldg(ops[0].twx, 2 * e), ldg(ops[0].twx, 2 * e + 1)};
+    }
+    const int c = wave * 8 + g;   // complex line of this lane: rows / columns 2c, 2c + 1
+    for (int o = blockIdx.x; o < nops; o += gridDim.x) {
+        const PlaneOp &op = ops[o];
+        const Tin *src = static_cast<const Tin *>(op.src);
+        for (int i = tid; i < N * N; i += NT)
+            Pl[(i >> 6) * PS + (i & 63)] = double(ldg(src, unsigned(i))) * pre_factor(op.epre, op.epre2, unsigned(i));
+        __syncthreads();
+        {   // z: rows x = 2c, 2c + 1
+            const int a = 2 * c * PS, b = a + PS;
+            cpx<double> v[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) v[m] = cpx<double>{Pl[a + p + 8 * m], Pl[b + p + 8 * m]};
+            plane_line(v, Pl, a, b, 1, TWz, op.Kz, p);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                Pl[a + p + 8 * m] = v[m].x;
+                Pl[b + p + 8 * m] = v[m].y;
+            }
+        }
+        __syncthreads();
+        {   // x: columns z = 2c, 2c + 1
+            const int a = 2 * c, b = a + 1;
+            cpx<double> v[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) v[m] = cpx<double>{Pl[a + (p + 8 * m) * PS], Pl[b + (p + 8 * m) * PS]};
+            plane_line(v, Pl, a, b, PS, TWx, op.Kx, p);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                Pl[a + (p + 8 * m) * PS] = v[m].x;
+                Pl[b + (p + 8 * m) * PS] = v[m].y;
+            }
+        }
+        __syncthreads();
+        Tout *dst = static_cast<Tout *>(op.dst);
+        for (int i = tid; i < N * N; i += NT) stg(dst, unsigned(i), Tout(Pl[(i >> 6) * PS + (i & 63)]));
+        __syncthreads();   // the next op overwrites the plane
+    }
+}
+
 // ------------------------------------------------------------ noise stream
 //
 // The reference's deviates (funGaussDev, SCFT.c:1769-1782): POSIX drand48,

@@ -211,9 +211,10 @@ struct Slot {
 };
 
 enum LaunchKind {
-    LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFTREG = 5, LK_FFTREG2 = 6
+    LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFTREG = 5, LK_FFTREG2 = 6,
+    LK_PLANE = 7
 };
-constexpr int kLaunchKinds = 7;
+constexpr int kLaunchKinds = 8;
 
 struct Launch {
     int kind;        // LaunchKind
@@ -263,6 +264,10 @@ struct scft_ctx {
     size_t lineops_cap = 0;
     DenseOp *denseops = nullptr;
     size_t denseops_cap = 0;
+    PlaneOp *planeops = nullptr;
+    size_t planeops_cap = 0;
+    bool plane_on = true;            // fused step on periodic 64 x 64 planes (SCFT_PLANE=0: off)
+    int nstep0 = 0;                  // launches of contour step 0
     PrepArgs *d_prep = nullptr;
     DenArgs *d_den = nullptr;
     UpdArgs *d_upd = nullptr;
@@ -686,13 +691,48 @@ int build_schedule(scft_ctx *c)
     for (int i = 0; i < c->nslots; ++i)
         if (c->slots[i].ready && c->slots[i].active) c->act.push_back(i);
     c->launches.clear();
+    c->nstep0 = 0;
     std::vector<LineOp> lops;
     std::vector<DenseOp> dops;
+    std::vector<PlaneOp> pops;
     int maxsteps = 0;
     for (int si : c->act)
         for (auto &ch : c->slots[si].chains) maxsteps = std::max(maxsteps, ch.nsteps);
     const size_t ts = c->tsize;
+    // every active slot a periodic 64 x 64 plane: one fused launch per step
+    bool plane = c->plane_on && c->fft_reg && c->zkind == Z_FFT && c->NY == 1 && c->NZ == kPlaneN;
+    for (int si : c->act) plane &= c->slots[si].NX == kPlaneN && !c->slots[si].p.mask;
     for (int t = 0; t < maxsteps; ++t) {
+        if (plane) {
+            Launch ln{};
+            ln.kind = LK_PLANE;
+            ln.pos = 3;
+            ln.L = kPlaneN;
+            ln.first = int(pops.size());
+            for (int si : c->act) {
+                Slot &s = c->slots[si];
+                const size_t V = s.V;
+                for (auto &ch : s.chains) {
+                    if (ch.nsteps <= t) continue;
+                    PlaneOp op{};
+                    op.src = static_cast<const char *>(ch.base) + (size_t)t * V * ts;
+                    op.dst = static_cast<char *>(ch.base) + (size_t)(t + 1) * V * ts;
+                    op.epre = s.E + (size_t)ch.step_field(t) * V;
+                    op.epre2 = slice_factor(s, ch, t);
+                    op.Kz = s.fz->K;
+                    op.Kx = s.fx->K;
+                    op.twz = reinterpret_cast<const double *>(s.fz->tw);
+                    op.twx = reinterpret_cast<const double *>(s.fx->tw);
+                    pops.push_back(op);
+                }
+            }
+            ln.count = int(pops.size()) - ln.first;
+            // persistent workgroups (36 KB of LDS each: four per CU)
+            ln.gx = std::max(1, std::min(ln.count, 4 * c->ncu));
+            if (ln.count > 0) c->launches.push_back(ln);
+            if (t == 0) c->nstep0 = int(c->launches.size());
+            continue;
+        }
         // axes in order z, y, x; every slot of the batch shares NY/NZ, so the
         // axis list is the same for all (x exists when any slot has NX > 1)
         std::vector<int> axes = {2};
@@ -843,9 +883,11 @@ int build_schedule(scft_ctx *c)
                 c->launches.push_back(ln);
             }
         }
+        if (t == 0) c->nstep0 = int(c->launches.size());
     }
     RETCHK(upload(c, &c->lineops, &c->lineops_cap, lops));
     RETCHK(upload(c, &c->denseops, &c->denseops_cap, dops));
+    RETCHK(upload(c, &c->planeops, &c->planeops_cap, pops));
     // per-slot argument arrays in active order
     std::vector<PrepArgs> prep;
     std::vector<DenArgs> den;
@@ -951,6 +993,10 @@ void launch_one(scft_ctx *c, const Launch &ln)
     dim3 g(ln.gx, ln.count);
     hipStream_t st = c->stream;
     switch (ln.kind) {
+    case LK_PLANE:
+        hipLaunchKernelGGL((k_plane64<Tin, Tout>), dim3(ln.gx), dim3(64 * kPlaneWaves), 0, st,
+                           c->planeops + ln.first, ln.count);
+        break;
     case LK_DENSE_MFMA: {
         const DenseOp *ops = c->denseops + ln.first;
         const dim3 g1(ln.gx), b(64 * kMfmaWaves);
@@ -1187,15 +1233,12 @@ int autotune_z(scft_ctx *c)
                 c->slots[t].active = true;
             }
             RETCHK(build_schedule(c));
-            // time the z launches of step 0 only
+            // time the launches of step 0 (the x launch is the same for both
+            // kinds unless the fused plane step replaces both axes)
             hipEvent_t e0, e1;
             HIPCHK(hipEventCreate(&e0));
             HIPCHK(hipEventCreate(&e1));
-            std::vector<Launch> zl;
-            for (auto &ln : c->launches) {
-                zl.push_back(ln);
-                break;
-            }
+            std::vector<Launch> zl(c->launches.begin(), c->launches.begin() + c->nstep0);
             std::swap(zl, c->launches);
             const bool f32 = c->tsize == 4;
             for (int w = 0; w < 3; ++w) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
@@ -1278,6 +1321,8 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     c->fft_wave = !(xf && std::strcmp(xf, "lds") == 0);
     c->fft_reg = !(xf && (std::strcmp(xf, "lds") == 0 || std::strcmp(xf, "wave") == 0));
     c->fft_reg_force = xf && std::strcmp(xf, "reg") == 0;
+    const char *pl = std::getenv("SCFT_PLANE");
+    c->plane_on = !(pl && pl[0] == '0');
     const char *dm = std::getenv("SCFT_DENSE");
     c->dense_mfma = !(dm && std::strcmp(dm, "valu") == 0);
     c->dense_mfma1 = dm && std::strcmp(dm, "mfma1") == 0;
@@ -1744,6 +1789,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     const double ts = double(c->tsize);
     for (const Launch &ln : c->launches) {
         const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
+                          ln.kind == LK_PLANE ||
                           ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFTREG ||
                             ln.kind == LK_FFTREG2) && ln.pos == 0);
         if ((kind == 0) != is_z) continue;
@@ -1757,7 +1803,17 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
         auto e_bytes = [](const double *e1, const double *e2, const double *ep) {
             return (e1 ? (e2 && e2 != e1 ? 16.0 : 8.0) : 0.0) + (ep ? 8.0 : 0.0);
         };
-        if (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) {
+        if (ln.kind == LK_PLANE) {
+            std::vector<PlaneOp> ops(ln.count);
+            HIPCHK(hipMemcpy(ops.data(), c->planeops + ln.first, ln.count * sizeof(PlaneOp), hipMemcpyDeviceToHost));
+            for (auto &op : ops) {
+                const double elems = double(kPlaneN) * kPlaneN;
+                U += elems;
+                B += elems * (in_b + out_b + e_bytes(op.epre, op.epre2, nullptr));
+                // both axes: 2 x 5 L log2 L per packed line pair, 32 pairs per axis
+                Fl += 2.0 * (kPlaneN / 2.0) * 2.0 * 5.0 * kPlaneN * std::log2(double(kPlaneN));
+            }
+        } else if (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) {
             std::vector<DenseOp> ops(ln.count);
             HIPCHK(hipMemcpy(ops.data(), c->denseops + ln.first, ln.count * sizeof(DenseOp), hipMemcpyDeviceToHost));
             for (auto &op : ops) {
@@ -1814,9 +1870,9 @@ int scft_get_schedule(const scft_ctx *c, char *buf, int len)
         if (ln.kind >= 0 && ln.kind < kLaunchKinds) n[ln.kind]++;
     std::snprintf(buf, len,
                   "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fftreg=%d dense_mfma=%d dense_mfma2=%d graph=%d "
-                  "fftreg2=%d",
+                  "fftreg2=%d plane=%d",
                   c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFTREG],
-                  n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0, n[LK_FFTREG2]);
+                  n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0, n[LK_FFTREG2], n[LK_PLANE]);
     return SCFT_OK;
 }
 
