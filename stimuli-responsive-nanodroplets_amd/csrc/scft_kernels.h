@@ -60,6 +60,13 @@ __device__ __forceinline__ T ldg(const T *p, unsigned i)
     return *gaddr(p, i);
 }
 
+// p[i] as a streaming (non-temporal) load: data read once per launch.
+template <typename T>
+__device__ __forceinline__ T ldg_nt(const T *p, unsigned i)
+{
+    return __builtin_nontemporal_load(gaddr(p, i));
+}
+
 template <typename T>
 __device__ __forceinline__ void stg(T *p, unsigned i, T v)
 {
@@ -77,6 +84,17 @@ __device__ __forceinline__ Pair<T> ldg_pair(const T *p, unsigned i)
 {
     typedef T v2 __attribute__((ext_vector_type(2)));
     const v2 v = *(const __attribute__((address_space(1))) v2 *)((const char *)p + i * (unsigned)sizeof(T));
+    return Pair<T>{v.x, v.y};
+}
+
+// The same as a streaming (non-temporal) load: for data read once per
+// launch, so that it does not displace reused operands from the caches.
+template <typename T>
+__device__ __forceinline__ Pair<T> ldg_pair_nt(const T *p, unsigned i)
+{
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load((const __attribute__((address_space(1))) v2 *)((const char *)p +
+                                                                                         i * (unsigned)sizeof(T)));
     return Pair<T>{v.x, v.y};
 }
 
@@ -560,7 +578,7 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
-        xa[r] = ldg_pair(src, a);
+        xa[r] = ldg_pair_nt(src, a);   // the slice is read once; the factors are reused every step
         ea[r] = ldg_pair(op.epre, a);
     }
     double sf[NT], af[NT];
@@ -1738,11 +1756,12 @@ __device__ __forceinline__ double simpson(const DenTerm &t, int V, int idx)
     const unsigned ui = unsigned(idx);
     const double qa = ldg(t.eqa, ui), qb = ldg(t.eqb, ui), da = ldg(t.eda, ui), db = ldg(t.edb, ui);
     auto fac = [](int s, int sw, double a, double b) { return s == 0 ? 1.0 : (s <= sw ? a : b); };
-    // slice base pointers are uniform (scalar arithmetic), the point offset is 32-bit
+    // slice base pointers are uniform (scalar arithmetic), the point offset is
+    // 32-bit; every slice is read once per iteration (streaming loads)
     auto qc = [&](int s) -> double {
         const int u = top - s;
-        return (double(ldg(q + (size_t)s * sV, ui)) * fac(s, t.qsw, qa, qb)) *
-               (double(ldg(qd + (size_t)u * sV, ui)) * fac(u, t.dsw, da, db));
+        return (double(ldg_nt(q + (size_t)s * sV, ui)) * fac(s, t.qsw, qa, qb)) *
+               (double(ldg_nt(qd + (size_t)u * sV, ui)) * fac(u, t.dsw, da, db));
     };
     if constexpr (sizeof(T) == 8) {
         // the reference's order: ends, then 4*odd, then 2*even (SCFT.c:2283-2297)
