@@ -6,7 +6,9 @@ transforms of a step in one workgroup per chain) against the oracle.
   and FP32-split to 1e-5, with the z operator forced to the transform;
 - a batch of three slots with different chi through the fused step against
   the same batch with the step split into its z and x launches;
-- SCFT_PLANE=0 turns the fused step off.
+- SCFT_PLANE=0 turns the fused step off; the split step then picks the
+  wave-per-line FFT for a launch of few lines and the register FFT for a
+  batch.
 """
 import os
 
@@ -107,3 +109,38 @@ def test_plane_batch_matches_split_step(abi, configs, lib, z_fft):
             assert _srel(fa, fb) <= 1e-12, (fa, fb)
     for a, b in zip(wf, ws):
         assert _rel(a, b) <= 1e-12
+
+
+def _kinds(sched):
+    return {k: int(v) for k, v in (kv.split("=") for kv in sched.split() if "=" in kv and kv[0] != "z")}
+
+
+def test_split_step_kind_follows_launch_size(abi, configs, lib, z_fft):
+    """With the fused step off, a C1 step's 64-point transforms go to the
+    wave-per-line FFT when the launch holds few lines (one solve) and to the
+    8 x 8 register FFT when it holds a batch (192 solves)."""
+    os.environ["SCFT_PLANE"] = "0"
+    try:
+        p = configs.params_c1()
+        one = abi.Batch(lib, p, 1, precision=abi.SCFT_FP32_SPLIT)
+        many = abi.Batch(lib, p, 192, precision=abi.SCFT_FP32_SPLIT)
+    finally:
+        del os.environ["SCFT_PLANE"]
+    w, pf = configs.random_fields(p, seed=1, amp=0.3, fields=["A", "B"])
+    F = []
+    for b in (one, many):
+        for s in range(b.nslots):
+            b.init_slot(s, p)
+            b.set_fields(s, w, pf)
+            b.activate(s)
+        F.append([o.free_energy for o in b.iterate()])
+    k1, kb = _kinds(one.schedule()), _kinds(many.schedule())
+    assert k1["plane"] == 0 and kb["plane"] == 0
+    assert k1["fftwave"] > 0 and k1["fftreg2"] == 0, one.schedule()
+    assert kb["fftreg2"] > 0 and kb["fftwave"] == 0, many.schedule()
+    # the 192 slots hold the same solve: identical results, and the single
+    # solve through the other transform within FP32-split rounding
+    assert all(f == F[1][0] for f in F[1])
+    assert _srel(F[0][0], F[1][0]) <= 1e-6, (F[0][0], F[1][0])
+    one.close()
+    many.close()
