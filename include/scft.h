@@ -206,6 +206,17 @@ int scft_slot_rng_state(scft_ctx *ctx, int slot, unsigned short xsubi[3]);
  * within its RELAXATION but that has no stream is an error (the reference
  * adds noise there).  outs[slot] is filled for active slots. */
 int scft_batch_iterate(scft_ctx *ctx, const unsigned long *iters, scft_iter_out *outs);
+/* scft_batch_iterate in two halves, so that the host can run other
+ * contexts meanwhile: scft_batch_launch enqueues the iteration on the
+ * context's stream and returns; scft_batch_complete waits for it and fills
+ * outs.  Two contexts launched one after the other run on the GPU
+ * concurrently (each has its own stream).  Between the two calls every
+ * other call on this context except scft_sync, scft_destroy and the
+ * read-only queries (scft_batch_size, scft_get_derived,
+ * scft_slot_get_derived, scft_get_schedule) returns SCFT_EINVAL.  Not in
+ * the reference: it serves solves of several chain structures at once. */
+int scft_batch_launch(scft_ctx *ctx, const unsigned long *iters);
+int scft_batch_complete(scft_ctx *ctx, scft_iter_out *outs);
 /* Device bytes one slot of a batch with prototype p needs (fields,
  * propagators, scratch); the serving layer admits solves by this footprint. */
 int scft_slot_bytes(const scft_params *p, int max_nx, int precision, size_t *bytes);

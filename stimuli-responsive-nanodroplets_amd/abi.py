@@ -169,6 +169,8 @@ def load_library(path=None):
         "scft_slot_get_densities": ([vp, C.c_int, dp, dp], C.c_int),
         "scft_slot_get_propagator": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, dp], C.c_int),
         "scft_batch_iterate": ([vp, P(C.c_ulong), P(ScftIterOut)], C.c_int),
+        "scft_batch_launch": ([vp, P(C.c_ulong)], C.c_int),
+        "scft_batch_complete": ([vp, P(ScftIterOut)], C.c_int),
         "scft_get_schedule": ([vp, C.c_char_p, C.c_int], C.c_int),
         "scft_slot_bytes": ([P(ScftParams), C.c_int, C.c_int, P(C.c_size_t)], C.c_int),
         "scft_device_memory": ([P(C.c_size_t), P(C.c_size_t)], C.c_int),
@@ -376,6 +378,21 @@ class Batch:
                     self._iters[i] = v
             ip = self._iters
         check(self.lib, self.lib.scft_batch_iterate(self.h, ip, self._outs), "scft_batch_iterate")
+        return self._outs
+
+    def launch(self, iters=None):
+        """Start one iteration of every active slot on the batch's stream and
+        return at once; complete() waits for it and returns the outs."""
+        ip = None
+        if iters is not None:
+            if iters is not self._iters:
+                for i, v in enumerate(iters):
+                    self._iters[i] = v
+            ip = self._iters
+        check(self.lib, self.lib.scft_batch_launch(self.h, ip), "scft_batch_launch")
+
+    def complete(self):
+        check(self.lib, self.lib.scft_batch_complete(self.h, self._outs), "scft_batch_complete")
         return self._outs
 
     def schedule(self):

@@ -268,6 +268,7 @@ struct scft_ctx {
     size_t planeops_cap = 0;
     bool plane_on = true;            // fused step on periodic 64 x 64 planes (SCFT_PLANE=0: off)
     int nstep0 = 0;                  // launches of contour step 0
+    bool inflight = false;           // a batched iteration launched by scft_batch_launch, not yet completed
     PrepArgs *d_prep = nullptr;
     DenArgs *d_den = nullptr;
     UpdArgs *d_upd = nullptr;
@@ -1134,7 +1135,7 @@ int enqueue(scft_ctx *c, int phases)
     return SCFT_OK;
 }
 
-int run_iteration(scft_ctx *c, float mf, const double *noise_dev)
+int run_iteration(scft_ctx *c, float mf, const double *noise_dev, bool wait = true)
 {
     if (c->dirty) RETCHK(build_schedule(c));
     RETCHK(set_iteration_args(c, mf, noise_dev));
@@ -1157,7 +1158,20 @@ int run_iteration(scft_ctx *c, float mf, const double *noise_dev)
     } else {
         RETCHK(enqueue(c, 1 | 2 | 4 | 8));
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (wait) HIPCHK(hipStreamSynchronize(c->stream));
+    return SCFT_OK;
+}
+
+// Entry points that read or change a context refuse while a batched
+// iteration started by scft_batch_launch runs on its stream: the schedule's
+// argument arrays and the slots' buffers belong to that launch until
+// scft_batch_complete.
+int refuse_inflight(const scft_ctx *c)
+{
+    if (c && c->inflight) {
+        set_err("a batched iteration is in flight (call scft_batch_complete first)");
+        return SCFT_EINVAL;
+    }
     return SCFT_OK;
 }
 
@@ -1367,6 +1381,7 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
 
 int scft_slot_init(scft_ctx *c, int slot, const scft_params *p)
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s || !p) return SCFT_EINVAL;
     s->active = false;
@@ -1379,6 +1394,7 @@ int scft_slot_init(scft_ctx *c, int slot, const scft_params *p)
 
 int scft_slot_activate(scft_ctx *c, int slot, int on)
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s) return SCFT_EINVAL;
     if (on && !s->ready) {
@@ -1437,6 +1453,7 @@ int scft_create(const scft_params *p, int precision, scft_ctx **out)
 int scft_destroy(scft_ctx *ctx)
 {
     if (!ctx) return SCFT_EINVAL;
+    if (ctx->inflight) (void)hipStreamSynchronize(ctx->stream);   // the launch still owns the buffers
     free_ctx(ctx);
     return SCFT_OK;
 }
@@ -1452,6 +1469,7 @@ int scft_get_derived(const scft_ctx *c, scft_derived *out) { return scft_slot_ge
 
 int scft_slot_set_fields(scft_ctx *c, int slot, const double *w, const double *p)
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s || !w || !p || !s->ready) return SCFT_EINVAL;
     HIPCHK(hipMemcpyAsync(s->w, w, sizeof(double) * 13 * s->V, hipMemcpyHostToDevice, c->stream));
@@ -1462,6 +1480,7 @@ int scft_slot_set_fields(scft_ctx *c, int slot, const double *w, const double *p
 
 int scft_slot_copy_fields(scft_ctx *c, int dst_slot, int src_slot)
 {
+    RETCHK(refuse_inflight(c));
     Slot *d = get_slot(c, dst_slot);
     Slot *s = get_slot(c, src_slot);
     if (!d || !s || !d->ready || !s->ready || d->V != s->V) {
@@ -1476,6 +1495,7 @@ int scft_slot_copy_fields(scft_ctx *c, int dst_slot, int src_slot)
 
 int scft_slot_get_fields(scft_ctx *c, int slot, double *w, double *p)
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s || !w || !p || !s->ready) return SCFT_EINVAL;
     HIPCHK(hipMemcpyAsync(w, s->w, sizeof(double) * 13 * s->V, hipMemcpyDeviceToHost, c->stream));
@@ -1486,6 +1506,7 @@ int scft_slot_get_fields(scft_ctx *c, int slot, double *w, double *p)
 
 int scft_slot_set_mask(scft_ctx *c, int slot, const double *mask)
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s || !mask || !s->ready) return SCFT_EINVAL;
     HIPCHK(hipMemcpyAsync(s->mask, mask, sizeof(double) * 3 * s->V, hipMemcpyHostToDevice, c->stream));
@@ -1507,6 +1528,7 @@ int scft_slot_set_mask(scft_ctx *c, int slot, const double *mask)
 
 int scft_slot_get_densities(scft_ctx *c, int slot, double *phi, double *phi_tot)
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s || !phi || !s->ready) return SCFT_EINVAL;
     HIPCHK(hipMemcpyAsync(phi, s->tot, sizeof(double) * 13 * s->V, hipMemcpyDeviceToHost, c->stream));
@@ -1518,6 +1540,7 @@ int scft_slot_get_densities(scft_ctx *c, int slot, double *phi, double *phi_tot)
 
 int scft_slot_get_propagator(scft_ctx *c, int slot, int sp, int which, int idx, double *out)
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s || !out || !s->ready) return SCFT_EINVAL;
     if (sp < 0 || sp >= SCFT_NSPECIES || s->d.ns_sp[sp] <= 0 || idx < 0 || idx >= s->d.ns_sp[sp]) {
@@ -1559,6 +1582,7 @@ int scft_get_propagator(scft_ctx *c, int sp, int which, int s, double *out)
 
 int scft_propagate(scft_ctx *c, unsigned long iter)
 {
+    RETCHK(refuse_inflight(c));
     if (!c) return SCFT_EINVAL;
     if (c->dirty) RETCHK(build_schedule(c));
     for (int si : c->act) c->slots[si].iter = iter;
@@ -1568,6 +1592,7 @@ int scft_propagate(scft_ctx *c, unsigned long iter)
 
 int scft_densities(scft_ctx *c, double *qsum, double *chempot)
 {
+    RETCHK(refuse_inflight(c));
     if (!c) return SCFT_EINVAL;
     if (c->dirty) RETCHK(build_schedule(c));
     RETCHK(enqueue(c, 2 | 8));
@@ -1631,6 +1656,7 @@ static int upload_noise(scft_ctx *c, float mf, const double *noise, const double
 
 int scft_update_fields(scft_ctx *c, unsigned long iter, float mf, const double *noise, scft_iter_out *out)
 {
+    RETCHK(refuse_inflight(c));
     (void)iter;
     if (!c) return SCFT_EINVAL;
     if (c->dirty) RETCHK(build_schedule(c));
@@ -1646,6 +1672,7 @@ int scft_update_fields(scft_ctx *c, unsigned long iter, float mf, const double *
 
 int scft_iterate(scft_ctx *c, unsigned long iter, float mf, const double *noise, scft_iter_out *out)
 {
+    RETCHK(refuse_inflight(c));
     if (!c) return SCFT_EINVAL;
     if (c->dirty) RETCHK(build_schedule(c));
     for (int si : c->act) c->slots[si].iter = iter;
@@ -1666,6 +1693,7 @@ int scft_iterate(scft_ctx *c, unsigned long iter, float mf, const double *noise,
 // (scft_slot_init) turns its stream off.
 int scft_slot_rng_seed(scft_ctx *c, int slot, const unsigned short xsubi[3])
 {
+    RETCHK(refuse_inflight(c));
     Slot *sp = get_slot(c, slot);
     if (!sp || !xsubi || !sp->ready) return SCFT_EINVAL;
     Slot &s = *sp;
@@ -1721,6 +1749,7 @@ int scft_slot_rng_seed(scft_ctx *c, int slot, const unsigned short xsubi[3])
 // The slot's drand48 state after its last completed iteration.
 int scft_slot_rng_state(scft_ctx *c, int slot, unsigned short xsubi[3])
 {
+    RETCHK(refuse_inflight(c));
     Slot *s = get_slot(c, slot);
     if (!s || !xsubi) return SCFT_EINVAL;
     if (!s->rng_on) {
@@ -1746,9 +1775,10 @@ int scft_rng_state(scft_ctx *c, unsigned short xsubi[3])
     return scft_slot_rng_state(c, 0, xsubi);
 }
 
-int scft_batch_iterate(scft_ctx *c, const unsigned long *iters, scft_iter_out *outs)
+int scft_batch_launch(scft_ctx *c, const unsigned long *iters)
 {
     if (!c) return SCFT_EINVAL;
+    RETCHK(refuse_inflight(c));
     if (c->dirty) RETCHK(build_schedule(c));
     for (int si : c->act) {
         Slot &s = c->slots[si];
@@ -1758,7 +1788,20 @@ int scft_batch_iterate(scft_ctx *c, const unsigned long *iters, scft_iter_out *o
             return SCFT_EINVAL;
         }
     }
-    RETCHK(run_iteration(c, -1.0f, nullptr));
+    RETCHK(run_iteration(c, -1.0f, nullptr, false));
+    c->inflight = true;
+    return SCFT_OK;
+}
+
+int scft_batch_complete(scft_ctx *c, scft_iter_out *outs)
+{
+    if (!c) return SCFT_EINVAL;
+    if (!c->inflight) {
+        set_err("no batched iteration in flight (scft_batch_launch)");
+        return SCFT_EINVAL;
+    }
+    c->inflight = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
     RETCHK(check_rng(c));
     if (outs)
         for (int si : c->act) {
@@ -1766,6 +1809,12 @@ int scft_batch_iterate(scft_ctx *c, const unsigned long *iters, scft_iter_out *o
             fill_out(c, si, (s.rng_on && s.iter <= s.p.relaxation) ? 1.0f : 0.0f, &outs[si]);
         }
     return SCFT_OK;
+}
+
+int scft_batch_iterate(scft_ctx *c, const unsigned long *iters, scft_iter_out *outs)
+{
+    RETCHK(scft_batch_launch(c, iters));
+    return scft_batch_complete(c, outs);
 }
 
 int scft_sync(scft_ctx *c)
@@ -1783,6 +1832,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
                        double *voxel_steps, int *nlaunch)
 {
     if (!c || reps < 1 || !avg_us || !bytes || !flops || !voxel_steps || !nlaunch) return SCFT_EINVAL;
+    RETCHK(refuse_inflight(c));
     if (c->dirty) RETCHK(build_schedule(c));
     std::vector<Launch> sel;
     double B = 0.0, Fl = 0.0, U = 0.0;

@@ -39,9 +39,9 @@ DRAND48_INITIAL = (0x330E, 0xABCD, 0x1234)   # POSIX drand48 state before any se
 
 class Request:
     __slots__ = ("rid", "params", "seed", "mask", "arrival", "slot", "admit_t", "done_t", "iters",
-                 "F", "status", "fields", "dw", "history", "keep_fields", "start_iter", "rng")
+                 "F", "status", "fields", "dw", "history", "keep_fields", "start_iter", "rng", "max_iters")
 
-    def __init__(self, rid, params, seed, mask=None, arrival=None, keep_fields=False, rng=None):
+    def __init__(self, rid, params, seed, mask=None, arrival=None, keep_fields=False, rng=None, max_iters=None):
         self.rid = rid
         self.params = params
         self.seed = seed              # (kind, arg)
@@ -59,6 +59,7 @@ class Request:
         self.keep_fields = keep_fields
         self.start_iter = 1           # number of the request's first iteration
         self.rng = rng                # drand48 state (3 x u16) of its noise stream, or None
+        self.max_iters = max_iters    # iteration budget of this request (None: the pool's)
 
     @property
     def time_to_converge(self):
@@ -104,6 +105,8 @@ class Pool:
         self.iterations = 0           # batched iterations run
         self.slot_iterations = 0      # instance-iterations run
         self.t_iterate = 0.0
+        self._inflight = False        # an iteration launched by step_launch, not yet completed
+        self._t_launch = 0.0
 
     # ------------------------------------------------------------ requests
     def submit(self, req):
@@ -170,16 +173,34 @@ class Pool:
     # ---------------------------------------------------------- iteration
     def step(self):
         """Admit, run one batched iteration, retire. Returns #active slots."""
+        self.step_launch()
+        return self.step_complete()
+
+    def step_launch(self):
+        """Admit and start one batched iteration on the pool's stream; returns
+        at once (step_complete waits for it).  Returns #slots launched."""
+        if self._inflight:
+            raise RuntimeError("step_launch before the previous iteration's step_complete")
         self._admit()
         if not self.running:
             return 0
         its = self.batch.iteration_numbers
         for s, req in self.running.items():
             its[s] = req.start_iter + req.iters
-        t0 = time.perf_counter()
-        outs = self.batch.iterate(its)
+        self._t_launch = time.perf_counter()
+        self.batch.launch(its)
+        self._inflight = True
+        return len(self.running)
+
+    def step_complete(self):
+        """Wait for the iteration step_launch started and retire converged,
+        failed and finished slots.  Returns #active slots."""
+        if not self._inflight:
+            return 0
+        self._inflight = False
+        outs = self.batch.complete()
         now = time.perf_counter()
-        self.t_iterate += now - t0
+        self.t_iterate += now - self._t_launch
         self.iterations += 1
         self.slot_iterations += len(self.running)
         for s in list(self.running):
@@ -192,7 +213,7 @@ class Pool:
                 self._retire(s, "failed", o, now)
             elif o.dw_rms < self.tol:
                 self._retire(s, "converged", o, now)
-            elif req.iters >= self.max_iters:
+            elif req.iters >= (req.max_iters or self.max_iters):
                 self._retire(s, "max_iters", o, now)
         return len(self.running)
 
@@ -284,11 +305,11 @@ class Server:
         return bool(self.pending) or any(p.busy() for p in self.pools.values())
 
     def step(self):
-        n = 0
-        for pool in self.pools.values():
-            if pool.busy():
-                n += pool.step()
-        return n
+        """One batched iteration of every busy pool: all are launched before
+        any is waited for, so the pools' iterations run on the GPU
+        concurrently (each batch has its own stream)."""
+        launched = [pool for pool in self.pools.values() if pool.busy() and pool.step_launch() > 0]
+        return sum(pool.step_complete() for pool in launched)
 
     def done(self):
         return [r for p in self.pools.values() for r in p.done]

@@ -41,3 +41,44 @@ def test_mixed_structures_stream_matches_alone(abi, configs, lib):
         assert a.status == m.status and a.iters == m.iters
         assert abs(a.F - m.F) <= 1e-12 * abs(a.F)
         alone.close()
+
+
+def test_launch_complete_runs_two_contexts_at_once(abi, configs, lib):
+    """scft_batch_launch on a 2D and a 3D batch, then scft_batch_complete on
+    both, gives the results of scft_batch_iterate on each alone, bit for bit;
+    while a launch is in flight the context refuses other calls."""
+    import serve
+    reqs = _requests(configs, serve)
+    p2, p3 = reqs[0].params, reqs[1].params
+
+    def make(p, seed):
+        b = abi.Batch(lib, p, 2, precision=abi.SCFT_FP64)
+        for s in range(2):
+            b.init_slot(s, p)
+            w, pf = configs.droplet_fields(p, radius=3.0, seed=seed + s)
+            b.set_fields(s, w, pf)
+            b.set_mask(s, configs.brush_mask(p))
+            b.activate(s)
+        return b
+
+    a2, a3 = make(p2, 0), make(p3, 5)
+    ref2 = [[o.free_energy for o in a2.iterate()][:2] for _ in range(4)]
+    ref3 = [[o.free_energy for o in a3.iterate()][:2] for _ in range(4)]
+    a2.close()
+    a3.close()
+    b2, b3 = make(p2, 0), make(p3, 5)
+    for it in range(4):
+        b2.launch()
+        b3.launch()
+        if it == 0:
+            with pytest.raises(abi.ScftError, match="in flight"):
+                b2.get_fields(0)
+            with pytest.raises(abi.ScftError, match="in flight"):
+                b3.launch()
+        f2 = [o.free_energy for o in b2.complete()][:2]
+        f3 = [o.free_energy for o in b3.complete()][:2]
+        assert f2 == ref2[it] and f3 == ref3[it], (it, f2, ref2[it], f3, ref3[it])
+    with pytest.raises(abi.ScftError, match="no batched iteration in flight"):
+        b2.complete()
+    b2.close()
+    b3.close()
