@@ -217,6 +217,11 @@ int scft_batch_iterate(scft_ctx *ctx, const unsigned long *iters, scft_iter_out 
  * the reference: it serves solves of several chain structures at once. */
 int scft_batch_launch(scft_ctx *ctx, const unsigned long *iters);
 int scft_batch_complete(scft_ctx *ctx, scft_iter_out *outs);
+/* Without waiting: 1 when the iteration started by scft_batch_launch has
+ * finished on the GPU (scft_batch_complete then returns at once), 0 while
+ * it runs, < 0 on error.  Lets a host loop advance each context at its own
+ * pace. */
+int scft_batch_query(scft_ctx *ctx);
 /* Device bytes one slot of a batch with prototype p needs (fields,
  * propagators, scratch); the serving layer admits solves by this footprint. */
 int scft_slot_bytes(const scft_params *p, int max_nx, int precision, size_t *bytes);

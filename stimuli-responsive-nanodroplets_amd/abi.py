@@ -171,6 +171,7 @@ def load_library(path=None):
         "scft_batch_iterate": ([vp, P(C.c_ulong), P(ScftIterOut)], C.c_int),
         "scft_batch_launch": ([vp, P(C.c_ulong)], C.c_int),
         "scft_batch_complete": ([vp, P(ScftIterOut)], C.c_int),
+        "scft_batch_query": ([vp], C.c_int),
         "scft_get_schedule": ([vp, C.c_char_p, C.c_int], C.c_int),
         "scft_slot_bytes": ([P(ScftParams), C.c_int, C.c_int, P(C.c_size_t)], C.c_int),
         "scft_device_memory": ([P(C.c_size_t), P(C.c_size_t)], C.c_int),
@@ -394,6 +395,13 @@ class Batch:
     def complete(self):
         check(self.lib, self.lib.scft_batch_complete(self.h, self._outs), "scft_batch_complete")
         return self._outs
+
+    def ready(self):
+        """True once the launched iteration has finished (complete() will not wait)."""
+        rc = self.lib.scft_batch_query(self.h)
+        if rc < 0:
+            check(self.lib, rc, "scft_batch_query")
+        return rc == 1
 
     def schedule(self):
         buf = C.create_string_buffer(256)

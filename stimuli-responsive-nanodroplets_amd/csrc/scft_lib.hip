@@ -1817,6 +1817,20 @@ int scft_batch_iterate(scft_ctx *c, const unsigned long *iters, scft_iter_out *o
     return scft_batch_complete(c, outs);
 }
 
+int scft_batch_query(scft_ctx *c)
+{
+    if (!c) return SCFT_EINVAL;
+    if (!c->inflight) {
+        set_err("no batched iteration in flight (scft_batch_launch)");
+        return SCFT_EINVAL;
+    }
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    set_err(std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    return SCFT_EHIP;
+}
+
 int scft_sync(scft_ctx *c)
 {
     if (!c) return SCFT_EINVAL;

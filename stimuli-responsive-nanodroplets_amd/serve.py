@@ -217,6 +217,14 @@ class Pool:
                 self._retire(s, "max_iters", o, now)
         return len(self.running)
 
+    @property
+    def inflight(self):
+        return self._inflight
+
+    def ready(self):
+        """True when the launched iteration has finished on the GPU."""
+        return self._inflight and self.batch.ready()
+
     def busy(self):
         return bool(self.running or self.queue)
 
@@ -311,6 +319,20 @@ class Server:
         launched = [pool for pool in self.pools.values() if pool.busy() and pool.step_launch() > 0]
         return sum(pool.step_complete() for pool in launched)
 
+    def advance(self):
+        """Without waiting: retire the iteration of every pool whose launch has
+        finished, and launch the next iteration of every idle pool with work,
+        so that each pool advances at its own pace (a fast 2D pool does not
+        wait for a slow 3D one).  Returns the number of running slots."""
+        n = 0
+        for pool in self.pools.values():
+            if pool.inflight and pool.ready():
+                pool.step_complete()
+            if not pool.inflight and pool.busy():
+                pool.step_launch()
+            n += len(pool.running)
+        return n
+
     def done(self):
         return [r for p in self.pools.values() for r in p.done]
 
@@ -330,7 +352,7 @@ class Server:
                 self.submit(r)
             if deadline is not None and now > deadline:
                 break
-            if self.step() == 0 and self.pending:
+            if self.advance() == 0 and self.pending:
                 time.sleep(min(0.001, max(0.0, self.pending[0][0] - now)))
 
     def close(self):

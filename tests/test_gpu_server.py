@@ -75,10 +75,17 @@ def test_launch_complete_runs_two_contexts_at_once(abi, configs, lib):
                 b2.get_fields(0)
             with pytest.raises(abi.ScftError, match="in flight"):
                 b3.launch()
+        if it == 1:   # poll without waiting until the 3D iteration is done
+            import time
+            t0 = time.perf_counter()
+            while not b3.ready():
+                assert time.perf_counter() - t0 < 60.0
         f2 = [o.free_energy for o in b2.complete()][:2]
         f3 = [o.free_energy for o in b3.complete()][:2]
         assert f2 == ref2[it] and f3 == ref3[it], (it, f2, ref2[it], f3, ref3[it])
     with pytest.raises(abi.ScftError, match="no batched iteration in flight"):
         b2.complete()
+    with pytest.raises(abi.ScftError, match="no batched iteration in flight"):
+        b2.ready()
     b2.close()
     b3.close()
