@@ -22,3 +22,56 @@ def test_iteration_bytes_match_the_accounting_table(abi, configs, lib):
     b3, t3 = bench.iteration_bytes(abi, lib, configs.params_c3(), 4.0)
     assert (t3["S"], t3["P"], t3["n"]) == (734, 521, 4)
     assert abs(b3 - 10.645e9) < 1e6
+
+
+class _FakePool:
+    """Stands in for serve.Pool in Server.advance: an iteration launched by
+    step_launch is ready after `polls` calls of ready()."""
+
+    def __init__(self, polls, work):
+        self.polls, self.work = polls, work
+        self.inflight, self.left = False, 0
+        self.running = {0: "r"}
+        self.launched = self.completed = 0
+
+    def busy(self):
+        return self.work > 0
+
+    def ready(self):
+        assert self.inflight
+        self.left -= 1
+        return self.left <= 0
+
+    def step_launch(self):
+        assert not self.inflight              # never two launches in flight
+        self.inflight, self.left = True, self.polls
+        self.launched += 1
+        return 1
+
+    def step_complete(self):
+        assert self.inflight
+        self.inflight = False
+        self.completed += 1
+        self.work -= 1
+        if self.work == 0:
+            self.running = {}
+        return len(self.running)
+
+
+def test_advance_lets_each_pool_run_at_its_own_pace():
+    """Server.advance completes whichever pool has finished and relaunches
+    it at once: a pool whose iterations take one poll runs five times as
+    many iterations as one whose iterations take five, and both finish all
+    their work."""
+    import serve
+    srv = serve.Server.__new__(serve.Server)
+    fast, slow = _FakePool(1, 50), _FakePool(5, 10)
+    srv.pools = {"2D": fast, "3D": slow}
+    srv.pending = []
+    for _ in range(25):
+        srv.advance()
+    assert fast.completed >= 4 * slow.completed > 0, (fast.completed, slow.completed)
+    while srv.busy():
+        srv.advance()
+    assert (fast.completed, slow.completed) == (50, 10)
+    assert not fast.inflight and not slow.inflight
