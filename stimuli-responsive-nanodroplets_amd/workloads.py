@@ -136,10 +136,11 @@ def run(lib, name, protos, precision, iters):
     z = b.time_launches(0, 5)
     x = b.time_launches(1, 5)
     print("%s slots=%d %s: %.3f ms per batched iteration, %.1f us per instance-iteration; "
-          "z launch %.1f us x%d (%.0f GB/s), last-axis launch %.1f us x%d; F[0]=%.10f; %s"
+          "z launch %.1f us x%d (%.0f GB/s), last-axis launch %.1f us x%d (%.0f GB/s); F[0]=%.10f; %s"
           % (name, len(protos), "fp32-split" if precision == abi.SCFT_FP32_SPLIT else "fp64", dt * 1e3,
              dt / len(protos) * 1e6, z["avg_us"], z["launches"], z["bytes"] / max(z["avg_us"], 1e-9) * 1e-3,
-             x["avg_us"], x["launches"], outs[0].free_energy, b.schedule()), flush=True)
+             x["avg_us"], x["launches"], x["bytes"] / max(x["avg_us"], 1e-9) * 1e-3, outs[0].free_energy,
+             b.schedule()), flush=True)
     b.close()
 
 
