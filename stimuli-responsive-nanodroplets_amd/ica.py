@@ -35,6 +35,14 @@ trial may settle on the saddle or escape from it depending on rounding-level
 asymmetries alone, so FP64 and FP32 runs could decide differently.  With
 the perturbation the escape is set by the seed, the same in both.
 
+FP64 polishing (`polish`, FP32-split runs only).  A converged trial whose
+F lies within `polish` of the current state's F is a close call.  Its
+state and the current one are re-converged in FP64 (a second pool, seeded
+with their converged fields), and the decision uses those two F values.
+A replacing trial then enters with its FP64 F and fields.  The FP32-split
+converged states are FP64 fixed points at the tolerance, so polishing
+takes few iterations.
+
 Neighbours: in-plane (i-1, j), (i+1, j), (i, j-1), (i, j+1) and cross-plane
 the same (i, j) on the previous and next plane of the cycle P0 -> P1 -> P2
 -> P0 (README.md:76-79).
@@ -91,7 +99,8 @@ class ICA:
     seed(point) -> the first-pass seed, a serve.Request seed tuple."""
 
     def __init__(self, lib, chi_ab, chi_ac, outdir, make_params=None, seed=None, precision=abi.SCFT_FP32_SPLIT,
-                 max_slots=256, tol=1e-5, max_iters=400000, delta=15.0, base=None, eps=1e-6, kick=1e-3):
+                 max_slots=256, tol=1e-5, max_iters=400000, delta=15.0, base=None, eps=1e-6, kick=1e-3,
+                 polish=None):
         self.lib = lib
         self.chi_ab = list(chi_ab)
         self.chi_ac = list(chi_ac)
@@ -114,6 +123,13 @@ class ICA:
         self.decisions = []        # (pass, point key, neighbour key, neighbour version, F_trial, F_before, replaced)
         self.passes = 0
         self.solves = 0
+        # FP64 polishing of close calls in the FP32-split mode: a converged
+        # trial whose F is within `polish` of the current state's is decided
+        # on both states re-converged in FP64 (seeded with their fields)
+        self.polish = polish if (polish and precision != abi.SCFT_FP64) else None
+        self.pool64 = None
+        self.f64 = {}              # (point key..., version) -> FP64 F of that state, once polished
+        self.polished = []         # (pass, point key, neighbour tag, F_trial, F_before, F64_trial, F64_current, replaced)
 
     def _default_params(self, chi_ab, chi_ac, plane):
         d = (0.0, self.delta, -self.delta)[plane]
@@ -166,6 +182,26 @@ class ICA:
         self.pool.run()
         self.solves += len(requests)
 
+    def _polish(self, items):
+        """Re-converge states in FP64 from their FP32-split fields; items are
+        (key, params, (w, p)); returns key -> (F, fields), (None, None) when
+        the FP64 solve does not converge."""
+        if self.pool64 is None:
+            proto = next(iter(self.points.values())).params
+            self.pool64 = serve.Pool(self.lib, proto, **dict(self._pool_args, precision=abi.SCFT_FP64))
+        reqs = [serve.Request(("polish",) + tuple(k), p, (serve.SEED_FIELDS, f), keep_fields=True)
+                for k, p, f in items]
+        for r in reqs:
+            self.pool64.submit(r)
+        self.pool64.run()
+        return {k: ((r.F, r.fields) if r.status == "converged" else (None, None)) for (k, _, _), r in zip(items, reqs)}
+
+    def _current_f64(self, pt):
+        key = pt.key + (pt.version,)
+        if key not in self.f64:
+            self.f64[key] = self._polish([(("c",) + key, pt.params, self._load_field(pt.field))])[("c",) + key][0]
+        return self.f64[key]
+
     def initial(self):
         """First pass: every point from its seed (README.md:86-95)."""
         reqs = []
@@ -198,21 +234,37 @@ class ICA:
         if not trials:
             return False
         self._run([t[3] for t in trials])
+        pol = {}
+        if self.polish:   # the close calls' trial states, re-converged in FP64 together
+            items = [(("t",) + pt.key + tag, pt.params, r.fields) for pt, nb, tag, r in trials
+                     if r.status == "converged" and abs(r.F - pt.F) < self.polish]
+            if items:
+                pol = self._polish(items)
         updated = False
         for pt, nb, tag, r in trials:
             pt.tried.add(tag)
             before = pt.F
             ok = r.status == "converged"
-            replace = ok and r.F < pt.F - self.eps   # FP64 comparison
+            tk = ("t",) + pt.key + tag
+            F_new, fields_new = r.F, r.fields
+            if ok and tk in pol and abs(r.F - pt.F) < self.polish:
+                F_new, fields_new = pol[tk]
+                F_cur = self._current_f64(pt)
+                replace = F_new is not None and F_cur is not None and F_new < F_cur - self.eps
+                self.polished.append((self.passes, pt.key, tag, r.F, before, F_new, F_cur, replace))
+            else:
+                replace = ok and r.F < pt.F - self.eps   # FP64 comparison
             self.decisions.append((self.passes, pt.key, nb.key, tag[3], r.F if ok else None, before, replace))
             if replace:
                 pt.history.append({"iteration": pt.iteration, "F": pt.F, "field": pt.field})
                 pt.version += 1
-                pt.F = r.F
+                pt.F = F_new
                 pt.iters = r.iters
                 pt.iteration = self.passes
                 pt.field = self._save_field("%s_%d_%d_v%d" % (PLANES[pt.plane], pt.i, pt.j, pt.version),
-                                            r.fields)
+                                            fields_new)
+                if tk in pol:
+                    self.f64[pt.key + (pt.version,)] = F_new
                 updated = True
         self.write_records()
         return updated
@@ -230,6 +282,8 @@ class ICA:
                 json.dump(pt.record(self.chi_ab[pt.i], self.chi_ac[pt.j]), f, indent=1)
 
     def close(self):
-        if self.pool is not None:
-            self.pool.close()
-            self.pool = None
+        for name in ("pool", "pool64"):
+            pool = getattr(self, name)
+            if pool is not None:
+                pool.close()
+                setattr(self, name, None)

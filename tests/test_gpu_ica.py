@@ -1,6 +1,7 @@
 """ICA phase-diagram workflow (README.md:59-128) on a small film: the same
-keep/replace decisions and free energies within 1e-6 in FP64 and in the
-FP32-split mode."""
+keep/replace decisions and free energies within 1e-6 in FP64, in the
+FP32-split mode, and in the FP32-split mode with its close calls polished
+in FP64."""
 import json
 import os
 
@@ -12,12 +13,13 @@ CHI_AB = (12.0, 16.0)
 CHI_AC = (20.0, 28.0)
 
 
-def _run(abi, lib, tmp_path, precision):
+def _run(abi, lib, tmp_path, precision, polish=None):
     import ica
     import serve
     base = dict(lx=12.0, lz=10.0, dr=1.0, ns=21, vol_dbk=0.2, vol_s1=0.05, len_s1=0.15, g1=0.02)
     w = ica.ICA(lib, CHI_AB, CHI_AC, str(tmp_path), precision=precision, max_slots=64, max_iters=60000,
-                base=base, delta=8.0, seed=lambda pt: (serve.SEED_DROPLET, {"radius": 4.0, "seed": 0}))
+                base=base, delta=8.0, seed=lambda pt: (serve.SEED_DROPLET, {"radius": 4.0, "seed": 0}),
+                polish=polish)
     w.run(max_passes=3)
     w.close()
     return w
@@ -46,3 +48,12 @@ def test_ica_decisions_fp64_vs_fp32_split(abi, lib, tmp_path):
     assert nrep == sum(d[6] for d in r64.decisions)
     for x in recs:
         assert os.path.exists(os.path.join(str(tmp_path / "fp64"), x["field"]))
+    # FP32-split with the close calls (within 1e-3 kT) polished in FP64: the
+    # same decisions, and the polished F values within 1e-6 of the FP64 run's
+    r32p = _run(abi, lib, tmp_path / "fp32p", abi.SCFT_FP32_SPLIT, polish=1e-3)
+    assert r32p.polished, "no close call polished"
+    print("polished", len(r32p.polished), "of", len(r32p.decisions))
+    assert [(d[0], d[1], d[2], d[3], d[6]) for d in r64.decisions] == \
+           [(d[0], d[1], d[2], d[3], d[6]) for d in r32p.decisions]
+    for k, pt in r64.points.items():
+        assert abs(pt.F - r32p.points[k].F) <= 1e-6, (k, pt.F, r32p.points[k].F)
