@@ -286,6 +286,7 @@ class Server:
         self.max_iters = max_iters
         self.pools = {}
         self.pending = []         # (arrival, seq, request) not yet arrived
+        self.progressed = False   # whether the last advance() completed or launched anything
         self.rejected = []
         self._seq = 0
         self.t0 = None
@@ -325,11 +326,14 @@ class Server:
         so that each pool advances at its own pace (a fast 2D pool does not
         wait for a slow 3D one).  Returns the number of running slots."""
         n = 0
+        self.progressed = False
         for pool in self.pools.values():
             if pool.inflight and pool.ready():
                 pool.step_complete()
+                self.progressed = True
             if not pool.inflight and pool.busy():
                 pool.step_launch()
+                self.progressed = True
             n += len(pool.running)
         return n
 
@@ -354,6 +358,8 @@ class Server:
                 break
             if self.advance() == 0 and self.pending:
                 time.sleep(min(0.001, max(0.0, self.pending[0][0] - now)))
+            elif not self.progressed:
+                time.sleep(50e-6)   # iterations take milliseconds: poll, do not spin
 
     def close(self):
         for p in self.pools.values():
