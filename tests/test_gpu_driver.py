@@ -138,5 +138,21 @@ def test_exact_restart_resumes_bit_exactly(abi, configs, tmp_path, name):
         assert np.array_equal(data[:13 * V].reshape(13, V), wg) and np.array_equal(data[13 * V:], pg)
 
 
+def test_driver_writes_hand_fixture_lines(abi, tmp_path):
+    """The drop-in driver's output files on the hand-solved state of
+    tests/golden/writers_expected.json equal the lines written out by hand
+    from the reference's printf formats (SCFT.c:1586-1597, 1614, 1645)."""
+    import paramsfile
+    from _cases import writers_case, check_writer_files
+    exe = os.path.join(BIN, "scft_run_writers")
+    assert os.path.exists(exe), "driver not built: run __graft_entry__.build()"
+    p, w, pf, g = writers_case(abi)
+    paramsfile.write_inputs(str(tmp_path), p, w, pf, None, counter=1, seed=(0, 0, 0))
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    check_writer_files(str(tmp_path), g)
+
+
 def lib_for_driver(abi):
     return abi.load_library()

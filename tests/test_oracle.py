@@ -165,6 +165,64 @@ def test_gauss_is_standard_normal(orc, configs):
     assert stats.kstest(x, "norm").pvalue > 1e-3
 
 
+def _glibc_drand48():
+    libc = C.CDLL("libc.so.6")
+    libc.drand48.restype = C.c_double
+    libc.seed48.restype = C.POINTER(C.c_ushort)
+    libc.seed48.argtypes = [C.POINTER(C.c_ushort)]
+    return libc
+
+
+def _polar(drand48):
+    """funGaussDev as SCFT.c:1769-1782 states it: uniform pairs on
+    (-1, 1)^2 until x1^2 + x2^2 < 1, then x1 * sqrt(-2 ln w / w)."""
+    while True:
+        x1 = 2.0 * drand48() - 1.0
+        x2 = 2.0 * drand48() - 1.0
+        w = x1 * x1 + x2 * x2
+        if not w >= 1.0:
+            break
+    return x1 * math.sqrt((-2.0 * math.log(w)) / w)
+
+
+@pytest.mark.parametrize("seed", [(0, 0, 0), (7, 8, 9), (65535, 4242, 31337)])
+def test_gauss_matches_polar_method_on_glibc_drand48(orc, configs, seed):
+    """The oracle's deviates equal the reference's polar method driven by
+    glibc's own drand48, bit for bit, and leave the same generator state
+    (what restart_random.dat records, SCFT.c:1652-1663)."""
+    libc = _glibc_drand48()
+    libc.seed48((C.c_ushort * 3)(*seed))
+    want = [_polar(libc.drand48) for _ in range(10000)]
+    st = orc.state(configs.t_bulk_small())
+    st.seed48(seed)
+    got = st.gauss_n(10000)
+    assert [float(x) for x in got] == want
+    old = libc.seed48((C.c_ushort * 3)(0, 0, 0))
+    assert tuple(old[i] for i in range(3)) == st.get_seed()
+
+
+def test_main_writer_lines_match_hand_fixture(orc, abi, tmp_path):
+    """orc_main's output files on a hand-solved state equal the lines
+    written out by hand from the reference's printf formats
+    (tests/golden/writers_expected.json: column order, widths, separators,
+    i-fastest voxel order).  restart_random.dat holds the state glibc's
+    drand48 reaches after the 5 x 16 deviates of the iteration."""
+    import paramsfile
+    from _cases import writers_case, check_writer_files
+    p, w, pf, g = writers_case(abi)
+    rc, d = orc.derive(p)
+    assert [round(d.alpha[4 + h], 14) for h in range(5)] == g["alpha"]
+    paramsfile.write_inputs(str(tmp_path), p, w, pf, None, counter=1, seed=(0, 0, 0))
+    assert orc.main(p, 1, 2, 1, str(tmp_path)) == 0
+    check_writer_files(str(tmp_path), g)
+    libc = _glibc_drand48()
+    libc.seed48((C.c_ushort * 3)(0, 0, 0))
+    for _ in range(5 * 16):
+        _polar(libc.drand48)
+    old = libc.seed48((C.c_ushort * 3)(0, 0, 0))
+    assert open(str(tmp_path / "restart_random.dat")).read() == "%d %d %d" % tuple(old[i] for i in range(3))
+
+
 # ------------------------------------------------------------- propagator
 
 def test_uniform_field_closed_form(orc, abi):
