@@ -8,6 +8,10 @@ scalars and a compact summary of the final fields.
 
     python tests/golden/make_golden.py        # c2_oracle.json
     python tests/golden/make_golden.py c1     # c1_oracle.json (C1, 2000 iterations)
+    python tests/golden/make_golden.py c3     # c3_oracle.json (3D droplet, 128x128x64, NS 201)
+    python tests/golden/make_golden.py c5     # c5_oracle.json (3D request, 96x96x48, NS 201)
+    python tests/golden/make_golden.py c4     # c4_oracle.json (two slots of the 192-slot sweep)
+    python tests/golden/make_golden.py ica    # ica_oracle.json (2x2x3 ICA grid, with and without the kick)
 """
 import json
 import os
@@ -96,8 +100,159 @@ def main():
         json.dump(out, f, indent=1)
 
 
+def main_3d(which):
+    """C3 (128x128x64, NS 201, short homo A) or the C5 3D request
+    (96x96x48, same chemistry): iterations 2 and 3 from the spherical-cap
+    droplet seed.  One C3 oracle iteration takes minutes."""
+    if which == "c3":
+        p, radius = configs.params_c3(), 6.0
+    else:
+        p, radius = configs.params_c5_3d(), 4.5
+    w, pf = configs.droplet_fields(p, radius=radius, seed=0)
+    name = "%s_cap_seed0" % which
+    out = {name: run(name, p, w, pf, configs.brush_mask(p), [2, 3])}
+    out[name]["recipe"] = ("configs.%s(); configs.droplet_fields(p, radius=%g, seed=0); configs.brush_mask(p)"
+                           % ("params_c3" if which == "c3" else "params_c5_3d", radius))
+    with open(os.path.join(HERE, "%s_oracle.json" % which), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def main_c4():
+    """Two slots of the phase-diagram sweep batch (workloads.c4_params(192),
+    seeded as tests/test_gpu_configs.py seeds them: droplet seed = slot % 64):
+    the first slot with NX 192 and the first with NX 320, iterations 2-4."""
+    import workloads
+    protos = workloads.c4_params(192)
+    nxs = [abi.grid_of(p)[0] for p in protos]
+    out = {}
+    for nx in (192, 320):
+        s = nxs.index(nx)
+        p = protos[s]
+        w, pf = configs.droplet_fields(p, radius=6.0, seed=s % 64)
+        name = "c4_slot%d_nx%d" % (s, nx)
+        out[name] = run(name, p, w, pf, configs.brush_mask(p), [2, 3, 4])
+        out[name]["slot"] = s
+        out[name]["recipe"] = ("workloads.c4_params(192)[%d]; configs.droplet_fields(p, radius=6.0, seed=%d); "
+                               "configs.brush_mask(p)" % (s, s % 64))
+    with open(os.path.join(HERE, "c4_oracle.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+# ------------------------------------------------------------------ ICA
+# The ICA workflow of stimuli-responsive-nanodroplets_amd/ica.py run with
+# the oracle solving every request (FP64, on the CPU): the decision fixture
+# tests/test_gpu_ica.py holds the GPU runs to.
+
+ICA_CHI_AB = (12.0, 16.0)
+ICA_CHI_AC = (20.0, 28.0)
+ICA_BASE = dict(lx=12.0, lz=10.0, dr=1.0, ns=21, vol_dbk=0.2, vol_s1=0.05, len_s1=0.15, g1=0.02)
+ICA_ARGS = dict(max_slots=64, max_iters=60000, delta=8.0)
+
+
+def _solve(job):
+    """One request to convergence in the oracle: (params bytes, w, p, mask,
+    tol, max_iters) -> (status, iters, F, dw, w, p)."""
+    pbytes, w, pf, mask, tol, max_iters = job
+    import math
+    params = abi.ScftParams.from_buffer_copy(pbytes)
+    st = oracle.Oracle(abi).state(params)
+    st.set_fields(w, pf)
+    if mask is not None:
+        st.set_mask(mask)
+    it, status, o = 0, "max_iters", None
+    while it < max_iters:
+        it += 1
+        o = st.iterate(it)
+        if not math.isfinite(o.free_energy) or not math.isfinite(o.dw_rms) or o.dw_rms > 1e3:
+            status = "failed"
+            break
+        if o.dw_rms < tol:
+            status = "converged"
+            break
+    wf, pfin = st.get_fields()
+    st.close()
+    return status, it, o.free_energy, o.dw_rms, wf, pfin
+
+
+class OraclePool:
+    """The subset of serve.Pool that ica.ICA uses, with every request solved
+    by the oracle in a process pool."""
+
+    def __init__(self, lib, proto, precision=None, max_slots=0, tol=1e-5, max_iters=10 ** 6, workers=8):
+        self.tol, self.max_iters, self.workers = tol, max_iters, workers
+        self.queue, self.done = [], []
+
+    def submit(self, req):
+        self.queue.append(req)
+
+    def run(self):
+        import multiprocessing as mp
+        jobs = []
+        for r in self.queue:
+            kind, arg = r.seed
+            if kind == "droplet":
+                w, pf = configs.droplet_fields(r.params, **(arg or {}))
+            elif kind == "fields":
+                w, pf = arg
+            else:
+                raise ValueError(kind)
+            mask = configs.brush_mask(r.params) if r.params.mask else None
+            jobs.append((bytes(r.params), w, pf, mask, self.tol, r.max_iters or self.max_iters))
+        with mp.get_context("fork").Pool(self.workers) as pool:
+            res = pool.map(_solve, jobs, chunksize=1)
+        for r, (status, it, F, dw, wf, pfin) in zip(self.queue, res):
+            r.status, r.iters, r.F, r.dw = status, it, F, dw
+            if r.keep_fields and status == "converged":
+                r.fields = (wf, pfin)
+        print("oracle pool: %d solves, iterations %s" % (len(res), sorted(x[1] for x in res)), flush=True)
+        self.done.extend(self.queue)
+        self.queue = []
+
+    def take_done(self):
+        out, self.done = self.done, []
+        return out
+
+    def close(self):
+        pass
+
+
+def ica_run(outdir, kick):
+    import ica
+    import serve
+    w = ica.ICA(None, ICA_CHI_AB, ICA_CHI_AC, outdir, precision=abi.SCFT_FP64, base=ICA_BASE, kick=kick,
+                seed=lambda pt: (serve.SEED_DROPLET, {"radius": 4.0, "seed": 0}), pool_factory=OraclePool,
+                **ICA_ARGS)
+    w.run(max_passes=3)
+    return w
+
+
+def main_ica():
+    """The 2 x 2 x 3 ICA grid of tests/test_gpu_ica.py in the oracle: with
+    the trial-seed perturbation (kick 1e-3, the module's default) and
+    without it (the README's plain neighbour seeding)."""
+    import tempfile
+    out = {"chi_ab": ICA_CHI_AB, "chi_ac": ICA_CHI_AC, "base": ICA_BASE, "args": ICA_ARGS, "max_passes": 3}
+    for name, kick in (("kick", 1e-3), ("no_kick", 0.0)):
+        w = ica_run(tempfile.mkdtemp(), kick)
+        out[name] = {"kick": kick, "passes": w.passes,
+                     "decisions": [[d[0], list(d[1]), list(d[2]), d[3], d[4], d[5], bool(d[6])] for d in w.decisions],
+                     "points": {"%d_%d_%d" % k: {"F": pt.F, "status": pt.status, "version": pt.version}
+                                for k, pt in w.points.items()}}
+        print(name, "passes", w.passes, "decisions", len(w.decisions),
+              "replaced", sum(d[6] for d in w.decisions), flush=True)
+    with open(os.path.join(HERE, "ica_oracle.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "c1":
+    mode = sys.argv[1] if len(sys.argv) > 1 else "c2"
+    if mode == "c1":
         main_c1()
+    elif mode in ("c3", "c5"):
+        main_3d(mode)
+    elif mode == "c4":
+        main_c4()
+    elif mode == "ica":
+        main_ica()
     else:
         main()

@@ -1,0 +1,192 @@
+"""The benchmark configurations beyond C1/C2 on the GPU, at their real
+sizes, against the oracle's FP64 fixtures (tests/golden/make_golden.py):
+
+- C3: 3D thin-film droplet, 128x128x64, NS 201, dbk AB + short homo A
+  (NS_A = 99) + homo C + brush S1 (NS_S1 = 19), MASK 1;
+- C5: the 3D served request, 96x96x48 with the same chemistry;
+- C4: the 192-slot phase-diagram sweep batch (NX 192/256/320 x 1 x 128 in
+  one batch), two fixture slots matched and identical slots bit-identical;
+- C5 stream: 256x1x128 and 96x96x48 requests arriving while others run,
+  each result equal to the same request alone.
+
+FP64 matches the oracle to 1e-10; FP32-split (FP32 propagator slices,
+FP64 arithmetic) is held to |dF| <= 1e-6 kT and the value printed.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-10
+FP32_DF = 1e-6
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.abs(a - b).max() / max(1e-300, np.abs(b).max()))
+
+
+def _srel(a, b):
+    return abs(a - b) / max(1.0, abs(b))
+
+
+def _golden(name):
+    with open(os.path.join(HERE, "golden", "%s_oracle.json" % name)) as f:
+        return json.load(f)
+
+
+def _check_rows(outs, g, precision_fp64, label):
+    for o, row in zip(outs, g["iters"]):
+        dF = abs(o.free_energy - row["F"])
+        print("%s iteration %d: F %.15g oracle %.15g |dF| %.3e" % (label, row["iter"], o.free_energy, row["F"], dF),
+              flush=True)
+        if precision_fp64:
+            assert _srel(o.free_energy, row["F"]) <= TOL, (row["iter"], o.free_energy, row["F"])
+            assert _srel(o.rmsdev, row["rmsdev"]) <= TOL
+            assert _srel(o.dw_rms, row["dw_rms"]) <= TOL
+            for sp in range(17):
+                assert _srel(o.chempot[sp], row["chempot"][sp]) <= TOL, (row["iter"], sp)
+                assert _srel(o.qsum[sp], row["qsum"][sp]) <= TOL, (row["iter"], sp)
+        else:
+            assert dF <= FP32_DF, (row["iter"], dF)
+            for sp in range(17):
+                assert abs(o.chempot[sp] - row["chempot"][sp]) <= FP32_DF, (row["iter"], sp)
+
+
+def _check_fields(wf, pfin, phi, g, precision_fp64):
+    idx = g["sample_idx"]
+    for f, vals in g["w_samples"].items():
+        if precision_fp64:
+            assert _rel(wf[int(f)][idx], vals) <= TOL, f
+            assert abs(np.sqrt((wf[int(f)] ** 2).sum()) - g["w_l2"][f]) <= TOL * g["w_l2"][f]
+        else:
+            assert np.abs(wf[int(f)][idx] - np.asarray(vals)).max() <= 1e-6, f
+    for f, vals in g["phi_samples"].items():
+        if precision_fp64:
+            assert _rel(phi[int(f)][idx], vals) <= TOL, f
+        else:
+            assert np.abs(phi[int(f)][idx] - np.asarray(vals)).max() <= 1e-6, f
+    if precision_fp64:
+        assert _rel(pfin[idx], g["p_samples"]) <= TOL
+
+
+@pytest.mark.parametrize("which", ["c3", "c5"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_3d_config_matches_golden(abi, configs, lib, which, prec):
+    """C3 (128x128x64, NS 201) and the C5 3D request (96x96x48, NS 201) from
+    the spherical-cap droplet: two iterations against the oracle's."""
+    key = "%s_cap_seed0" % which
+    g = _golden(which)[key]
+    p = configs.params_c3() if which == "c3" else configs.params_c5_3d()
+    radius = 6.0 if which == "c3" else 4.5
+    precision = abi.SCFT_FP64 if prec == "fp64" else abi.SCFT_FP32_SPLIT
+    print("case", which, prec, abi.grid_of(p), flush=True)
+    ctx = abi.Context(lib, p, precision)
+    print(ctx.schedule(), flush=True)
+    d = ctx.derived
+    assert (d.nx, d.ny, d.nz) == ((128, 128, 64) if which == "c3" else (96, 96, 48))
+    assert d.ns_sp[4] == 99 and d.ns_sp[14] == 19 and d.ns_sp[0] == 201   # homo A, S1, dbk1
+    w, pf = configs.droplet_fields(p, radius=radius, seed=0)
+    ctx.set_fields(w, pf)
+    ctx.set_mask(configs.brush_mask(p))
+    outs = [ctx.iterate(row["iter"]) for row in g["iters"]]
+    _check_rows(outs, g, prec == "fp64", "%s %s" % (which, prec))
+    wf, pfin = ctx.get_fields()
+    phi, _ = ctx.get_densities()
+    _check_fields(wf, pfin, phi, g, prec == "fp64")
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_c4_sweep_batch_matches_golden(abi, configs, lib, prec):
+    """The whole 192-slot C4 batch (chi_AB N 15..36 x chi_AC N 20..55, NX
+    192/256/320 drawn per slot, one batch, one launch per contour step over
+    every slot): the NX 192 and NX 320 fixture slots match the oracle and
+    slots with the same parameters and seed agree bit for bit."""
+    import workloads
+    g = _golden("c4")
+    protos = workloads.c4_params(192)
+    precision = abi.SCFT_FP64 if prec == "fp64" else abi.SCFT_FP32_SPLIT
+    b = abi.Batch(lib, protos[0], 192, max_nx=320, precision=precision)
+    for s, p in enumerate(protos):
+        b.init_slot(s, p)
+        w, pf = configs.droplet_fields(p, radius=6.0, seed=s % 64)
+        b.set_fields(s, w, pf)
+        b.set_mask(s, configs.brush_mask(p))
+        b.activate(s)
+    print(b.schedule(), flush=True)
+    iters = [row["iter"] for row in next(iter(g.values()))["iters"]]
+    # Batch.iterate returns its reused out array: keep a copy per iteration
+    outs = [[abi.ScftIterOut.from_buffer_copy(o) for o in b.iterate([it] * 192)] for it in iters]
+    for key, fx in g.items():
+        s = fx["slot"]
+        assert abi.grid_of(protos[s])[0] == int(key.split("nx")[1])
+        _check_rows([o[s] for o in outs], fx, prec == "fp64", "c4 %s slot %d" % (prec, s))
+        wf, pfin = b.get_fields(s)
+        phi, _ = b.get_densities(s)
+        _check_fields(wf, pfin, phi, fx, prec == "fp64")
+    # identical parameters and seeds -> identical results, wherever the slot sits
+    groups = {}
+    for s, p in enumerate(protos):
+        groups.setdefault((p.lx, p.chi[0][1], p.chi[0][2], s % 64), []).append(s)
+    dups = [v for v in groups.values() if len(v) > 1]
+    assert dups
+    for v in dups:
+        for o in outs:
+            assert all(o[s].free_energy == o[v[0]].free_energy for s in v), v
+        w0, _ = b.get_fields(v[0])
+        for s in v[1:]:
+            assert np.array_equal(b.get_fields(s)[0], w0), v
+    b.close()
+
+
+def _stream(configs, serve):
+    """Config-5 requests at full size: 2D 256/192/320 x 1 x 128 (C2
+    chemistry, chi from the C4 grid) and 3D 96x96x48 (C3 chemistry), each
+    with a small iteration budget, arriving while others run."""
+    spec = [("2d", 32.0, 25.0, 40.0, 0.0, 6), ("3d", 12.0, 25.0, 40.0, 0.0, 3), ("2d", 24.0, 18.0, 30.0, 0.01, 7),
+            ("2d", 40.0, 33.0, 50.0, 0.02, 5), ("3d", 12.0, 30.0, 45.0, 0.03, 4), ("2d", 32.0, 21.0, 55.0, 0.05, 6),
+            ("2d", 24.0, 36.0, 20.0, 0.08, 4)]
+    out = []
+    for k, (kind, lx, cab, cac, t, n) in enumerate(spec):
+        if kind == "2d":
+            p = configs.chem_droplet(lx=lx, chi_ab=cab, chi_ac=cac)
+        else:
+            p = configs.params_c5_3d(chi_ab=cab, chi_ac=cac)
+        out.append(serve.Request(k, p, (serve.SEED_DROPLET, {"radius": 6.0 if kind == "2d" else 4.5, "seed": k}),
+                                 arrival=t, max_iters=n))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_c5_stream_full_size_matches_alone(abi, configs, lib, prec):
+    """2D and 3D requests at their real sizes through serve.Server (one pool
+    per structure, requests admitted as they arrive): each result equals
+    the same request solved alone: iterations and status exactly, F to
+    1e-12 relative in FP64 and 1e-9 in FP32-split (a pool of one slot may
+    pick another transform kernel, whose FP64 rounding can flip the
+    rounding of an FP32 slice value)."""
+    import serve
+    precision = abi.SCFT_FP64 if prec == "fp64" else abi.SCFT_FP32_SPLIT
+    srv = serve.Server(lib, precision=precision, default_slots=4, max_nx=320, tol=0.0)
+    reqs = _stream(configs, serve)
+    srv.run_stream(reqs, deadline=120.0)
+    assert len(srv.pools) == 2
+    done = {r.rid: r for r in srv.done()}
+    assert sorted(done) == list(range(len(reqs)))
+    srv.close()
+    for r in _stream(configs, serve):
+        alone = serve.Pool(lib, r.params, precision=precision, max_slots=1, max_nx=320, tol=0.0)
+        r.arrival = None
+        alone.submit(r)
+        alone.run()
+        a = alone.done[0]
+        m = done[r.rid]
+        print("request", r.rid, abi.grid_of(r.params), m.status, m.iters, "F %.15g alone %.15g" % (m.F, a.F), flush=True)
+        assert a.status == m.status == "max_iters" and a.iters == m.iters == r.max_iters
+        assert abs(a.F - m.F) <= (1e-12 if prec == "fp64" else 1e-9) * abs(a.F)
+        alone.close()
