@@ -100,8 +100,9 @@ class ICA:
 
     def __init__(self, lib, chi_ab, chi_ac, outdir, make_params=None, seed=None, precision=abi.SCFT_FP32_SPLIT,
                  max_slots=256, tol=1e-5, max_iters=400000, delta=15.0, base=None, eps=1e-6, kick=1e-3,
-                 polish=None):
+                 polish=None, pool_factory=None):
         self.lib = lib
+        self.pool_factory = pool_factory or serve.Pool
         self.chi_ab = list(chi_ab)
         self.chi_ac = list(chi_ac)
         self.outdir = outdir
@@ -176,10 +177,11 @@ class ICA:
     def _run(self, requests):
         if self.pool is None:
             proto = next(iter(self.points.values())).params
-            self.pool = serve.Pool(self.lib, proto, **self._pool_args)
+            self.pool = self.pool_factory(self.lib, proto, **self._pool_args)
         for r in requests:
             self.pool.submit(r)
         self.pool.run()
+        self.pool.take_done()    # the caller holds the requests; the pool keeps none
         self.solves += len(requests)
 
     def _polish(self, items):
@@ -188,15 +190,17 @@ class ICA:
         the FP64 solve does not converge."""
         if self.pool64 is None:
             proto = next(iter(self.points.values())).params
-            self.pool64 = serve.Pool(self.lib, proto, **dict(self._pool_args, precision=abi.SCFT_FP64))
+            self.pool64 = self.pool_factory(self.lib, proto, **dict(self._pool_args, precision=abi.SCFT_FP64))
         reqs = [serve.Request(("polish",) + tuple(k), p, (serve.SEED_FIELDS, f), keep_fields=True)
                 for k, p, f in items]
         for r in reqs:
             self.pool64.submit(r)
         self.pool64.run()
+        self.pool64.take_done()
         return {k: ((r.F, r.fields) if r.status == "converged" else (None, None)) for (k, _, _), r in zip(items, reqs)}
 
     def _current_f64(self, pt):
+        """FP64 F of the point's current state (polished on demand)."""
         key = pt.key + (pt.version,)
         if key not in self.f64:
             self.f64[key] = self._polish([(("c",) + key, pt.params, self._load_field(pt.field))])[("c",) + key][0]
@@ -235,11 +239,25 @@ class ICA:
             return False
         self._run([t[3] for t in trials])
         pol = {}
-        if self.polish:   # the close calls' trial states, re-converged in FP64 together
-            items = [(("t",) + pt.key + tag, pt.params, r.fields) for pt, nb, tag, r in trials
-                     if r.status == "converged" and abs(r.F - pt.F) < self.polish]
+        if self.polish:
+            # the close calls against the states at the start of the pass: the
+            # trial states and the current states they compete with,
+            # re-converged in FP64 together in one batch
+            items, seen = [], set()
+            for pt, nb, tag, r in trials:
+                if r.status != "converged" or abs(r.F - pt.F) >= self.polish:
+                    continue
+                items.append((("t",) + pt.key + tag, pt.params, r.fields))
+                ck = pt.key + (pt.version,)
+                if ck not in self.f64 and ck not in seen:
+                    seen.add(ck)
+                    items.append((("c",) + ck, pt.params, self._load_field(pt.field)))
             if items:
-                pol = self._polish(items)
+                for k, v in self._polish(items).items():
+                    if k[0] == "c":
+                        self.f64[k[1:]] = v[0]
+                    else:
+                        pol[k] = v
         updated = False
         for pt, nb, tag, r in trials:
             pt.tried.add(tag)
@@ -247,7 +265,12 @@ class ICA:
             ok = r.status == "converged"
             tk = ("t",) + pt.key + tag
             F_new, fields_new = r.F, r.fields
-            if ok and tk in pol and abs(r.F - pt.F) < self.polish:
+            # closeness is judged against the point's state now: an earlier
+            # trial of this pass may have replaced it
+            close = bool(self.polish) and ok and abs(r.F - pt.F) < self.polish
+            if close:
+                if tk not in pol:   # a close call only against a state replaced in this pass
+                    pol.update(self._polish([(tk, pt.params, r.fields)]))
                 F_new, fields_new = pol[tk]
                 F_cur = self._current_f64(pt)
                 replace = F_new is not None and F_cur is not None and F_new < F_cur - self.eps
@@ -263,7 +286,7 @@ class ICA:
                 pt.iteration = self.passes
                 pt.field = self._save_field("%s_%d_%d_v%d" % (PLANES[pt.plane], pt.i, pt.j, pt.version),
                                             fields_new)
-                if tk in pol:
+                if close:   # the new state's F is an FP64 re-converged value
                     self.f64[pt.key + (pt.version,)] = F_new
                 updated = True
         self.write_records()

@@ -149,6 +149,8 @@ class Pool:
             s = self.free_slots.popleft()
             self.batch.init_slot(s, req.params)
             self._seed_fields(req, s)
+            if req.seed[0] == SEED_FIELDS:
+                req.seed = (SEED_FIELDS, None)   # the slot holds the fields now
             if req.params.relaxation > 0 or req.rng is not None:
                 self.batch.rng_seed(s, req.rng if req.rng is not None else DRAND48_INITIAL)
             self.batch.activate(s, True)
@@ -216,6 +218,17 @@ class Pool:
             elif req.iters >= (req.max_iters or self.max_iters):
                 self._retire(s, "max_iters", o, now)
         return len(self.running)
+
+    def take_done(self):
+        """Return the finished requests and forget them (the pool keeps no
+        reference, so their fields can be freed), except those a queued
+        request still waits on as its seed (SEED_FROM)."""
+        out, self.done = self.done, []
+        needed = {r.seed[1] for r in self.queue if r.seed[0] == SEED_FROM}
+        for r in out:
+            if r.rid not in needed:
+                self.by_id.pop(r.rid, None)
+        return out
 
     @property
     def inflight(self):
@@ -295,6 +308,11 @@ class Server:
         key = structure_key(req.params)
         pool = self.pools.get(key)
         if pool is None:
+            # a new pool times its z-operator candidates when it is created:
+            # finish the iterations in flight first, so they do not compete
+            for other in self.pools.values():
+                if other.inflight:
+                    other.step_complete()
             nx = grid_of(req.params)[0]
             pool = Pool(self.lib, req.params, precision=self.precision, max_slots=self.slots.get(key, self.default_slots),
                         max_nx=max(self.max_nx, nx), hbm_fraction=self.hbm_fraction, tol=self.tol,
@@ -332,10 +350,19 @@ class Server:
                 pool.step_complete()
                 self.progressed = True
             if not pool.inflight and pool.busy():
-                pool.step_launch()
-                self.progressed = True
+                if pool.step_launch() > 0:
+                    self.progressed = True
             n += len(pool.running)
         return n
+
+    def stuck(self):
+        """True when requests wait but none can ever start: nothing in
+        flight, nothing running, and every queue head waits on a seed
+        request that cannot finish (SEED_FROM on a request not yet run)."""
+        if self.pending:
+            return False
+        return any(p.queue for p in self.pools.values()) and \
+            not any(p.inflight or p.running for p in self.pools.values())
 
     def done(self):
         return [r for p in self.pools.values() for r in p.done]
@@ -356,7 +383,10 @@ class Server:
                 self.submit(r)
             if deadline is not None and now > deadline:
                 break
-            if self.advance() == 0 and self.pending:
+            n = self.advance()
+            if self.stuck():
+                raise RuntimeError("serving stream stuck: queued requests wait on seeds that cannot run")
+            if n == 0 and self.pending:
                 time.sleep(min(0.001, max(0.0, self.pending[0][0] - now)))
             elif not self.progressed:
                 time.sleep(50e-6)   # iterations take milliseconds: poll, do not spin

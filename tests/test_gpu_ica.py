@@ -1,7 +1,8 @@
 """ICA phase-diagram workflow (README.md:59-128) on a small film: the same
-keep/replace decisions and free energies within 1e-6 in FP64, in the
-FP32-split mode, and in the FP32-split mode with its close calls polished
-in FP64."""
+keep/replace decisions and free energies within 1e-6 as the oracle's run of
+the same workflow (tests/golden/ica_oracle.json, every solve in the CPU
+oracle), in FP64, in the FP32-split mode, and in the FP32-split mode with
+its close calls polished in FP64."""
 import json
 import os
 
@@ -25,9 +26,33 @@ def _run(abi, lib, tmp_path, precision, polish=None):
     return w
 
 
+def _oracle_fixture():
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ica_oracle.json")) as f:
+        return json.load(f)
+
+
+def _check_against_oracle(run, fx, label):
+    """Decisions (pass, point, neighbour, neighbour version, replaced) equal
+    the oracle's; trial and point free energies within 1e-6 kT."""
+    want = [(d[0], tuple(d[1]), tuple(d[2]), d[3], d[6]) for d in fx["decisions"]]
+    got = [(d[0], tuple(d[1]), tuple(d[2]), d[3], bool(d[6])) for d in run.decisions]
+    assert got == want, label
+    for d, o in zip(run.decisions, fx["decisions"]):
+        assert (d[4] is None) == (o[4] is None)
+        if d[4] is not None:
+            assert abs(d[4] - o[4]) <= 1e-6, (label, d, o)
+    for k, pt in run.points.items():
+        o = fx["points"]["%d_%d_%d" % k]
+        assert pt.status == o["status"] and pt.version == o["version"], (label, k)
+        assert abs(pt.F - o["F"]) <= 1e-6, (label, k, pt.F, o["F"])
+
+
 def test_ica_decisions_fp64_vs_fp32_split(abi, lib, tmp_path):
+    fx = _oracle_fixture()["kick"]
     r64 = _run(abi, lib, tmp_path / "fp64", abi.SCFT_FP64)
+    _check_against_oracle(r64, fx, "fp64")
     r32 = _run(abi, lib, tmp_path / "fp32", abi.SCFT_FP32_SPLIT)
+    _check_against_oracle(r32, fx, "fp32-split")
     for r in (r64, r32):
         assert all(pt.status == "converged" for pt in r.points.values())
         assert r.passes >= 1
@@ -57,3 +82,4 @@ def test_ica_decisions_fp64_vs_fp32_split(abi, lib, tmp_path):
            [(d[0], d[1], d[2], d[3], d[6]) for d in r32p.decisions]
     for k, pt in r64.points.items():
         assert abs(pt.F - r32p.points[k].F) <= 1e-6, (k, pt.F, r32p.points[k].F)
+    _check_against_oracle(r32p, fx, "fp32-split polished")

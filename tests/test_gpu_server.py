@@ -89,3 +89,47 @@ def test_launch_complete_runs_two_contexts_at_once(abi, configs, lib):
         b2.ready()
     b2.close()
     b3.close()
+
+
+def _oracle_solve(orc, configs, p, seed, tol, max_iters):
+    """The oracle's solve of one request: iterations from 1 until ||dw|| < tol."""
+    st = orc.state(p)
+    w, pf = configs.droplet_fields(p, radius=3.0, seed=seed)
+    st.set_fields(w, pf)
+    st.set_mask(configs.brush_mask(p))
+    for it in range(1, max_iters + 1):
+        o = st.iterate(it)
+        if o.dw_rms < tol:
+            break
+    F = o.free_energy
+    st.close()
+    return it, F
+
+
+def test_served_requests_admitted_mid_flight_match_oracle(abi, configs, orc, lib):
+    """A two-slot FP64 pool with five requests (chi and LX differ): three
+    wait in the queue and are admitted as earlier ones converge and free
+    their slots, into a batch whose other slot is mid-solve.  Each request
+    converges after the same number of iterations as the oracle's solve of
+    it, to the oracle's F within 1e-9."""
+    import serve
+    specs = [(12.0, 12.0), (14.0, 10.0), (16.0, 12.0), (13.0, 8.0), (15.0, 12.0)]
+    reqs = []
+    for k, (chi, lx) in enumerate(specs):
+        p = configs.chem_droplet(lx=lx, lz=10.0, dr=1.0, ns=21, vol_dbk=0.2, vol_s1=0.05, len_s1=0.15, g1=0.02,
+                                 chi_ab=chi)
+        reqs.append(serve.Request(k, p, (serve.SEED_DROPLET, {"radius": 3.0, "seed": k})))
+    pool = serve.Pool(lib, reqs[0].params, precision=abi.SCFT_FP64, max_slots=2, max_nx=12, tol=1e-5,
+                      max_iters=20000)
+    assert pool.nslots == 2
+    for r in reqs:
+        pool.submit(r)
+    pool.run()
+    admitted_mid = [r for r in reqs if r.admit_t > min(q.done_t for q in reqs)]
+    assert len(admitted_mid) >= 3
+    for r in reqs:
+        it, F = _oracle_solve(orc, configs, r.params, r.rid, 1e-5, 20000)
+        print("request", r.rid, r.status, r.iters, "F %.15g oracle %d %.15g" % (r.F, it, F), flush=True)
+        assert r.status == "converged" and r.iters == it
+        assert abs(r.F - F) <= 1e-9
+    pool.close()
