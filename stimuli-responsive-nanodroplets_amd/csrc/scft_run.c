@@ -225,6 +225,15 @@ int main(void)
 {
     scft_params p;
     params_from_macros(&p);
+    /* SCFT_DUMP_PARAMS=path: write the parameter struct taken from the
+     * compiled header and stop (lets tests compare header readers) */
+    const char *dump = getenv("SCFT_DUMP_PARAMS");
+    if (dump) {
+        FILE *f = fopen(dump, "wb");
+        if (!f || fwrite(&p, sizeof(p), 1, f) != 1) die(dump);
+        fclose(f);
+        return 0;
+    }
 
     /* volume check, SCFT.c:84-101 */
     const double volumeCheck = VOL_DBK_1 + VOL_DBK_2 + VOL_DBK_3 + VOL_DBK_4 + VOL_HOMOPOLYMER_A + VOL_HOMOPOLYMER_B +

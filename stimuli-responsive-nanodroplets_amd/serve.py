@@ -256,6 +256,26 @@ class Pool:
         self.batch.close()
 
 
+def request_from_run_dir(rid, directory, header="parameters.h", **kw):
+    """A request for the run a reference user prepared in `directory`: its
+    parameters.h (read at run time, paramsfile.parse), counter.dat,
+    restart_fields.dat and, for MASK 1, wMask.dat.  Iterations are numbered
+    on from counter.dat, and the budget is the reference's
+    (counter + 1 ... ITER + 1, SCFT.c:1088-1090) unless max_iters is given."""
+    import os
+    import paramsfile
+    p, run = paramsfile.load(os.path.join(directory, header))
+    counter = int(open(os.path.join(directory, "counter.dat")).read().split()[0])
+    if counter == 0:
+        raise ValueError("counter.dat = 0: the reference starts such a run from clock-seeded fields; "
+                         "give the request a seed instead")
+    w, pf, mask = paramsfile.read_inputs(directory, p)
+    kw.setdefault("max_iters", max(1, run["iter"] - counter + 1))
+    r = Request(rid, p, (SEED_FIELDS, (w, pf)), mask=mask, **kw)
+    r.start_iter = counter + 1
+    return r
+
+
 def percentiles(values, qs=(50, 99)):
     if not values:
         return {q: None for q in qs}
