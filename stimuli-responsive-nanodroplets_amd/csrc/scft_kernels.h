@@ -216,6 +216,7 @@ struct DenseOp {
     const void *src;
     void *dst;
     const double *epre, *epre2, *epost;
+    const float *e2f;    // E^2 of epre's field in FP32 (inside a block, FP32-split), or null
     const double *MsT;   // [hs][ldm]: MsT[j][k] = Ms[k][j]
     const double *MaT;   // [ha][ldm]
     int n, hs, ha, ldm;
@@ -570,16 +571,27 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
     const int cc = c + 1 < n ? c : 0;       // load column (pairs past the row read column 0)
     const int nr = min(16, op.nrows - rbase);
     const Tin *src = static_cast<const Tin *>(op.src);
-    // factor: E alone (start slice), E^2 inside a block, E E2 at a block boundary
-    const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? 2 : 3);
+    // factor: E alone (start slice), E^2 inside a block (FP64 E squared, or
+    // the FP32 E^2 array: mode 4), E E2 at a block boundary
+    const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? (op.e2f ? 4 : 2) : 3);
     Pair<Tin> xa[16];
     Pair<double> ea[16];
     const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(cc);
+    if (mode == 4) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
-        xa[r] = ldg_pair_nt(src, a);   // the slice is read once; the factors are reused every step
-        ea[r] = ldg_pair(op.epre, a);
+        for (int r = 0; r < 16; ++r) {
+            const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
+            xa[r] = ldg_pair_nt(src, a);   // the slice is read once; the factors are reused every step
+            const Pair<float> f = ldg_pair(op.e2f, a);
+            ea[r] = Pair<double>{double(f.x), double(f.y)};
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
+            xa[r] = ldg_pair_nt(src, a);
+            ea[r] = ldg_pair(op.epre, a);
+        }
     }
     double sf[NT], af[NT];
 #pragma unroll
@@ -1622,6 +1634,7 @@ __global__ void k_rng_commit(const RngSlot *__restrict__ slots)
 struct PrepArgs {
     const double *w;     // 13*V
     double *E;           // 13*V
+    float *E2f;          // 13*V: E^2 rounded once to FP32 (FP32-split z operator), or null
     void *q0_dbk1;       // dbk1 q slice 0 (T), or null
     int fields[13];      // fields whose Boltzmann factor is needed
     int nfields;
@@ -1631,6 +1644,30 @@ struct PrepArgs {
     int seed_mode;
     int nx, ny, nz, delta;
 };
+
+// E^2 rounded to FP32 without bias.  Round-to-nearest gives every voxel
+// of a region where w is uniform the same rounding error, which the z
+// operator then applies at every step of a block: a systematic relative
+// error of up to NS x 2^-25 in q there.  Here the value goes to one of its
+// two FP32 neighbours with probability proportional to its closeness, the
+// choice made by a hash of the array offset: the rounding errors have zero
+// mean and no correlation between voxels, so averages over voxels (Q, the
+// free energy) do not accumulate them.
+__device__ __forceinline__ float round_unbiased_f32(double x, unsigned key)
+{
+    const float f = float(x);
+    const float lo = (double(f) <= x) ? f : nextafterf(f, 0.0f);
+    const float hi = (double(f) <= x) ? nextafterf(f, 3.0e38f) : f;
+    if (double(lo) == x) return lo;
+    unsigned h = key * 0x9E3779B1u;              // integer hash (multiply-xorshift)
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    const double u = (double(h) + 0.5) * (1.0 / 4294967296.0);
+    return (u * (double(hi) - double(lo)) < x - double(lo)) ? hi : lo;
+}
 
 // grid: (blocks, instances)
 template <typename T>
@@ -1642,7 +1679,9 @@ __global__ void __launch_bounds__(kThreads) k_prep(const PrepArgs *__restrict__ 
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < a.V; idx += stride) {
         for (int i = 0; i < a.nfields; ++i) {
             const size_t o = (size_t)a.fields[i] * a.V + idx;
-            a.E[o] = exp(a.ds * (-a.w[o] / 2.0));
+            const double e = exp(a.ds * (-a.w[o] / 2.0));
+            a.E[o] = e;
+            if (a.E2f) a.E2f[o] = round_unbiased_f32(e * e, unsigned(o));
         }
         if (a.seed_mode == 2) {
             q0[idx] = T(1);

@@ -157,6 +157,10 @@ struct DenseTable {
 
 enum ZKind { Z_FFT = 0, Z_DENSE = 1 };
 
+// Smallest grid (points per instance) whose FP32-split z operator reads
+// E^2 as FP32.
+constexpr int kE2fMinV = 16384;
+
 // Process-wide autotuner results: (n, dirichlet, alpha bits, storage,
 // slots timed) -> ZKind.
 std::mutex g_tune_mu;
@@ -190,6 +194,7 @@ struct Slot {
     bool ready = false, active = false;
     unsigned long iter = 0;
     double *w = nullptr, *pf = nullptr, *tot = nullptr, *phitot = nullptr, *mask = nullptr, *E = nullptr;
+    float *E2f = nullptr;   // E^2 in FP32 for the z operator inside a block (FP32-split), or null
     void *q[SCFT_NSPECIES] = {};
     void *qd[SCFT_NSPECIES] = {};
     double *qpart = nullptr, *dpart = nullptr, *upart = nullptr;
@@ -253,6 +258,7 @@ struct scft_ctx {
     bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
     bool dense_mfma1 = false;        // force the one-wave-per-SIMD MFMA kernel for even NZ too
     bool tmp_wide = false;           // FP64 scratch between axis operators in FP32 mode
+    bool e2f = false;                // FP32 E^2 for the MFMA z operator inside a block (FP32 mode)
     int ncu = 256;                   // compute units of the device (persistent launches)
     Scalars *scal = nullptr;         // [nslots] device
     Scalars *scal_host = nullptr;    // [nslots] pinned
@@ -407,6 +413,7 @@ int slot_alloc(scft_ctx *c, Slot &s)
     RETCHK(ctx_alloc(c, &s.phitot, V));
     RETCHK(ctx_alloc(c, &s.mask, 3 * V));
     RETCHK(ctx_alloc(c, &s.E, 13 * V));
+    if (c->e2f) RETCHK(ctx_alloc(c, &s.E2f, 13 * V));
     for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
         const int ns = d.ns_sp[sp];
         if (ns <= 0) continue;
@@ -541,6 +548,7 @@ int slot_init(scft_ctx *c, Slot &s, const scft_params &p)
         if (used[f]) pa.fields[pa.nfields++] = f;
     pa.w = s.w;
     pa.E = s.E;
+    pa.E2f = s.E2f;
     pa.q0_dbk1 = d.ns_sp[0] > 0 ? s.q[0] : nullptr;
     pa.ds = DS;
     pa.V = s.V;
@@ -765,6 +773,8 @@ int build_schedule(scft_ctx *c)
                         op.epre = ep;
                         op.epre2 = ep2;
                         op.epost = nullptr;
+                        // inside a block the step's factor is E^2 of one field
+                        op.e2f = (s.E2f && ep && ep2 == ep) ? s.E2f + (size_t)ch.step_field(t) * V : nullptr;
                         op.MsT = s.dz->MsT;
                         op.MaT = s.dz->MaT;
                         op.n = s.dz->n;
@@ -1342,6 +1352,14 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     c->dense_mfma1 = dm && std::strcmp(dm, "mfma1") == 0;
     const char *tw = std::getenv("SCFT_TMP");
     c->tmp_wide = tw && std::strcmp(tw, "fp64") == 0;
+    // SCFT_E2F=1 (FP32-split, grids of at least kE2fMinV points): the MFMA
+    // z operator reads E^2 as FP32 inside a block, 4 instead of 8 bytes per
+    // voxel-step.  Off by default: at C2 with 192 slots it cut the z
+    // launch's bytes from 319 to 241 MB but its time only from 77.5 to
+    // 74.6 us (the batched iteration 14.39 -> 14.18 ms), while the per-voxel
+    // rounding of E^2 moves F by up to ~1e-6 kT on small grids.
+    const char *e2 = std::getenv("SCFT_E2F");
+    c->e2f = precision == SCFT_FP32_SPLIT && e2 && e2[0] == '1' && V >= size_t(kE2fMinV);
     {
         int dev = 0, ncu = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -1418,6 +1436,7 @@ int scft_slot_bytes(const scft_params *p, int max_nx, int precision, size_t *byt
     const size_t V = size_t(std::max(max_nx, d.nx)) * d.ny * d.nz;
     const size_t ts = precision == SCFT_FP64 ? 8 : 4;
     size_t b = 44 * V * sizeof(double);                 // w, p, tot, phitot, mask, E
+    if (precision == SCFT_FP32_SPLIT) b += 13 * V * sizeof(float);   // FP32 E^2 (at most)
     size_t nchains = 0;
     for (int sp = 0; sp < SCFT_NSPECIES; ++sp) {
         if (d.ns_sp[sp] <= 0) continue;
@@ -1864,7 +1883,8 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
         // the scratch between axes T (FP64 when wide)
         const double in_b = (ln.pos == 0 || ln.pos == 3 || !c->tmp_wide) ? ts : 8.0;
         const double out_b = (ln.pos == 2 || ln.pos == 3 || !c->tmp_wide) ? ts : 8.0;
-        auto e_bytes = [](const double *e1, const double *e2, const double *ep) {
+        auto e_bytes = [](const double *e1, const double *e2, const double *ep, const float *e2f = nullptr) {
+            if (e2f) return 4.0 + (ep ? 8.0 : 0.0);
             return (e1 ? (e2 && e2 != e1 ? 16.0 : 8.0) : 0.0) + (ep ? 8.0 : 0.0);
         };
         if (ln.kind == LK_PLANE) {
@@ -1883,7 +1903,8 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
             for (auto &op : ops) {
                 const double elems = double(op.nrows) * op.n;
                 U += elems;
-                B += elems * (in_b + out_b + e_bytes(op.epre, op.epre2, op.epost));
+                B += elems * (in_b + out_b +
+                              e_bytes(op.epre, op.epre2, op.epost, ln.kind == LK_DENSE_MFMA2 ? op.e2f : nullptr));
                 Fl += 2.0 * op.nrows * (double(op.hs) * op.hs + double(op.ha) * op.ha);
             }
         } else {
@@ -1941,3 +1962,4 @@ int scft_get_schedule(const scft_ctx *c, char *buf, int len)
 }
 
 }  // extern "C"
+
