@@ -7,6 +7,8 @@
  * (run.txt, energies.dat, free_energy.dat, deviations.dat,
  * densities_dbk_<1e8+iter>.dat, counter.dat, restart_*.dat), with the
  * iteration loop running on the GPU through the C ABI of include/scft.h.
+ * Snapshots are copied to host buffers at their iteration and written out
+ * by a thread of their own while the GPU runs on.
  *
  * Replaces SCFT.c:82-1664.  Environment: SCFT_PRECISION=fp32 selects the
  * FP32-split propagators (default FP64, the reference's arithmetic).
@@ -162,6 +164,58 @@ static int read_exact_restart(unsigned long iter, size_t V, double *w, double *p
     return ok;
 }
 
+/* One snapshot (SCFT.c:1601-1664) written by a thread of its own while the
+ * GPU runs the next iterations: the densities file, counter.dat,
+ * restart_fields.dat (+ the full-precision sidecar) and restart_random.dat,
+ * in the reference's order, from host copies taken at the snapshot
+ * iteration. */
+typedef struct {
+    unsigned long iter;
+    int nx, ny, nz, exact;
+    double *w, *pf, *phi;      /* owned copies, 13V / V / 13V */
+    unsigned short rng[3];
+} snap_job;
+
+static void *snap_thread(void *arg)
+{
+    const snap_job *j = (const snap_job *)arg;
+    const size_t V = (size_t)j->nx * j->ny * j->nz;
+    char name[64];
+    snprintf(name, sizeof(name), "%s%d.dat", DENSITY, (int)(100000000 + j->iter));
+    FILE *fid = fopen(name, "w");
+    if (!fid) die(name);
+    for (int k = 0; k < j->nz; k++)
+        for (int jj = 0; jj < j->ny; jj++)
+            for (int i = 0; i < j->nx; i++) {
+                size_t idx = k + (size_t)j->nz * (jj + (size_t)j->ny * i);
+                fprintf(fid, "%5d %5d %5d", i, jj, k);
+                for (int f = 0; f < 13; f++) fprintf(fid, " %10.6lf", j->phi[(size_t)f * V + idx]);
+                for (int f = 0; f < 13; f++) fprintf(fid, " %12.6lf", j->w[(size_t)f * V + idx]);
+                fprintf(fid, " %12.6lf \n", j->pf[idx]);
+            }
+    fclose(fid);
+    fid = fopen(COUNTER, "w");
+    if (!fid) die(COUNTER);
+    fprintf(fid, "%lu", j->iter);
+    fclose(fid);
+    fid = fopen(RESTART_FIELDS, "w");
+    if (!fid) die(RESTART_FIELDS);
+    for (int k = 0; k < j->nz; k++)
+        for (int jj = 0; jj < j->ny; jj++)
+            for (int i = 0; i < j->nx; i++) {
+                size_t idx = k + (size_t)j->nz * (jj + (size_t)j->ny * i);
+                for (int f = 0; f < 13; f++) fprintf(fid, "%lf ", j->w[(size_t)f * V + idx]);
+                fprintf(fid, "%lf \n", j->pf[idx]);
+            }
+    fclose(fid);
+    if (j->exact) write_exact_restart(j->iter, V, j->w, j->pf);
+    fid = fopen(RESTART_RANDOM, "w");
+    if (!fid) die(RESTART_RANDOM);
+    fprintf(fid, "%hu %hu %hu", j->rng[0], j->rng[1], j->rng[2]);
+    fclose(fid);
+    return NULL;
+}
+
 static void check(int rc, const char *what)
 {
     if (rc != SCFT_OK) {
@@ -261,19 +315,23 @@ int main(void)
     check(scft_create(&p, (prec && strcmp(prec, "fp32") == 0) ? SCFT_FP32_SPLIT : SCFT_FP64, &ctx), "scft_create");
 
     double *w = calloc(13 * V, sizeof(double)), *pf = calloc(V, sizeof(double));
-    double *phi = calloc(13 * V, sizeof(double));
-    if (!w || !pf || !phi) die("calloc");
+    if (!w || !pf) die("calloc");
     if (MASK) {                                                       /* SCFT.c:756-781 */
         double *mask = calloc(3 * V, sizeof(double));
         fid = fopen(W_MASK, "r");
         if (!fid) die("Error opening the w field mask file");
+        size_t got = 0;
         for (int k = 0; k < NZv; k++)
             for (int j = 0; j < NYv; j++)
                 for (int i = 0; i < NXv; i++) {
                     size_t idx = k + (size_t)NZv * (j + (size_t)NYv * i);
-                    if (fscanf(fid, "%lf %lf %lf", &mask[idx], &mask[V + idx], &mask[2 * V + idx]) != 3) break;
+                    if (fscanf(fid, "%lf %lf %lf", &mask[idx], &mask[V + idx], &mask[2 * V + idx]) != 3) goto mask_done;
+                    got++;
                 }
+    mask_done:
         fclose(fid);
+        if (got < V)   /* the reference reads on the same way; say so */
+            fprintf(stderr, "warning: %s holds %zu of %zu voxel lines; the rest stay 0\n", W_MASK, got, V);
         check(scft_set_mask(ctx, mask), "scft_set_mask");
         free(mask);
     }
@@ -296,6 +354,7 @@ int main(void)
     } else {                                                          /* SCFT.c:844-879 */
         fid = fopen(RESTART_FIELDS, "r");
         if (!fid) die("Error opening restart fields file");
+        size_t got = 0;
         for (int k = 0; k < NZv; k++)
             for (int j = 0; j < NYv; j++)
                 for (int i = 0; i < NXv; i++) {
@@ -303,11 +362,15 @@ int main(void)
                     double v[14];
                     if (fscanf(fid, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &v[0], &v[1], &v[2], &v[3],
                                &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12], &v[13]) != 14)
-                        break;
+                        goto fields_done;
                     for (int f = 0; f < 13; f++) w[(size_t)f * V + idx] = v[f];
                     pf[idx] = v[13];
+                    got++;
                 }
+    fields_done:
         fclose(fid);
+        if (got < V)
+            fprintf(stderr, "warning: %s holds %zu of %zu voxel lines; the rest stay 0\n", RESTART_FIELDS, got, V);
         fid = fopen(RESTART_RANDOM, "r");
         if (!fid) die("Error opening restart random file");
         if (fscanf(fid, "%hu %hu %hu", &iSeed[0], &iSeed[1], &iSeed[2]) != 3) die("restart_random.dat");
@@ -324,6 +387,21 @@ int main(void)
     const int exact = exact_env && strcmp(exact_env, "0") != 0;
     const size_t ndraw = (size_t)d.n_active * V;
     double *noise = host_rng ? calloc(ndraw ? ndraw : 1, sizeof(double)) : NULL;
+    /* snapshots: two host buffers, each written out by its own thread */
+    snap_job jobs[2];
+    for (int b = 0; b < 2; b++) {
+        jobs[b].nx = NXv;
+        jobs[b].ny = NYv;
+        jobs[b].nz = NZv;
+        jobs[b].exact = exact;
+        jobs[b].w = calloc(13 * V, sizeof(double));
+        jobs[b].pf = calloc(V, sizeof(double));
+        jobs[b].phi = calloc(13 * V, sizeof(double));
+        if (!jobs[b].w || !jobs[b].pf || !jobs[b].phi) die("calloc");
+    }
+    pthread_t writer;
+    int writing = 0;
+    unsigned long nsnap = 0;
     while (iIter <= ITER) {                                           /* SCFT.c:1088 */
         iIter++;
         const float MF = (iIter > RELAXATION) ? 0.0f : 1.0f;           /* SCFT.c:1515-1522 */
@@ -357,47 +435,30 @@ int main(void)
             fclose(fid);
         }
         if (fmod((double)iIter, D_WRT) == 0 || iIter == 1) {          /* SCFT.c:1601-1664 */
-            check(scft_get_fields(ctx, w, pf), "scft_get_fields");
-            check(scft_get_densities(ctx, phi, NULL), "scft_get_densities");
-            char name[64];
-            snprintf(name, sizeof(name), "%s%d.dat", DENSITY, (int)(100000000 + iIter));
-            fid = fopen(name, "w");
-            if (!fid) die(name);
-            for (int k = 0; k < NZv; k++)
-                for (int j = 0; j < NYv; j++)
-                    for (int i = 0; i < NXv; i++) {
-                        size_t idx = k + (size_t)NZv * (j + (size_t)NYv * i);
-                        fprintf(fid, "%5d %5d %5d", i, j, k);
-                        for (int f = 0; f < 13; f++) fprintf(fid, " %10.6lf", phi[(size_t)f * V + idx]);
-                        for (int f = 0; f < 13; f++) fprintf(fid, " %12.6lf", w[(size_t)f * V + idx]);
-                        fprintf(fid, " %12.6lf \n", pf[idx]);
-                    }
-            fclose(fid);
-            fid = fopen(COUNTER, "w");
-            fprintf(fid, "%lu", iIter);
-            fclose(fid);
-            fid = fopen(RESTART_FIELDS, "w");
-            for (int k = 0; k < NZv; k++)
-                for (int j = 0; j < NYv; j++)
-                    for (int i = 0; i < NXv; i++) {
-                        size_t idx = k + (size_t)NZv * (j + (size_t)NYv * i);
-                        for (int f = 0; f < 13; f++) fprintf(fid, "%lf ", w[(size_t)f * V + idx]);
-                        fprintf(fid, "%lf \n", pf[idx]);
-                    }
-            fclose(fid);
-            if (exact) write_exact_restart(iIter, V, w, pf);
-            unsigned short cur[3];
-            if (host_rng) lcg_get(&gen, cur);
-            else check(scft_rng_state(ctx, cur), "scft_rng_state");
-            fid = fopen(RESTART_RANDOM, "w");
-            fprintf(fid, "%hu %hu %hu", cur[0], cur[1], cur[2]);
-            fclose(fid);
+            /* the previous snapshot's writer must be done with its buffers */
+            if (writing) {
+                pthread_join(writer, NULL);
+                writing = 0;
+            }
+            snap_job *j = &jobs[nsnap++ & 1];
+            j->iter = iIter;
+            check(scft_get_fields(ctx, j->w, j->pf), "scft_get_fields");
+            check(scft_get_densities(ctx, j->phi, NULL), "scft_get_densities");
+            if (host_rng) lcg_get(&gen, j->rng);
+            else check(scft_rng_state(ctx, j->rng), "scft_rng_state");
+            if (pthread_create(&writer, NULL, snap_thread, j) == 0) writing = 1;
+            else snap_thread(j);
         }
+    }
+    if (writing) pthread_join(writer, NULL);
+    for (int b = 0; b < 2; b++) {
+        free(jobs[b].w);
+        free(jobs[b].pf);
+        free(jobs[b].phi);
     }
     free(noise);
     free(w);
     free(pf);
-    free(phi);
     scft_destroy(ctx);
     return 0;
 }

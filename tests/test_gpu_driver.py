@@ -99,7 +99,9 @@ def test_exact_restart_resumes_bit_exactly(abi, configs, tmp_path, name):
     full = tmp_path / "full"
     full.mkdir()
     paramsfile.write_inputs(str(full), p, w, pf, mask, counter=1, seed=(7, 8, 9))
-    env = dict(os.environ, SCFT_EXACT_RESTART="1")
+    # one z operator in every process (the autotuner times both on a small
+    # grid and may pick either, which changes the last bits)
+    env = dict(os.environ, SCFT_EXACT_RESTART="1", SCFT_ZKIND="dense")
     r = subprocess.run([exe], cwd=str(full), capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stderr
     last = int(open(str(full / "counter.dat")).read())
@@ -127,7 +129,11 @@ def test_exact_restart_resumes_bit_exactly(abi, configs, tmp_path, name):
         it, V = np.frombuffer(raw[8:24].tobytes(), dtype=np.uint64)
         data = np.frombuffer(raw[24:].tobytes(), dtype=np.float64)
         assert it == last and data.size == 14 * V
-        ctx = abi.Context(lib_for_driver(abi), p, abi.SCFT_FP64)
+        os.environ["SCFT_ZKIND"] = "dense"
+        try:
+            ctx = abi.Context(lib_for_driver(abi), p, abi.SCFT_FP64)
+        finally:
+            del os.environ["SCFT_ZKIND"]
         ctx.set_fields(w, pf)
         if mask is not None:
             ctx.set_mask(mask)
