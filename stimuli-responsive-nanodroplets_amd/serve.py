@@ -39,7 +39,8 @@ DRAND48_INITIAL = (0x330E, 0xABCD, 0x1234)   # POSIX drand48 state before any se
 
 class Request:
     __slots__ = ("rid", "params", "seed", "mask", "arrival", "slot", "admit_t", "done_t", "iters",
-                 "F", "status", "fields", "dw", "history", "keep_fields", "start_iter", "rng", "max_iters")
+                 "F", "status", "fields", "dw", "history", "keep_fields", "start_iter", "rng", "max_iters",
+                 "F64", "dF", "dw64")
 
     def __init__(self, rid, params, seed, mask=None, arrival=None, keep_fields=False, rng=None, max_iters=None):
         self.rid = rid
@@ -60,6 +61,9 @@ class Request:
         self.start_iter = 1           # number of the request's first iteration
         self.rng = rng                # drand48 state (3 x u16) of its noise stream, or None
         self.max_iters = max_iters    # iteration budget of this request (None: the pool's)
+        self.F64 = None               # FP64 free energy of the converged state (Pool fp64_check)
+        self.dF = None                # |F - F64|
+        self.dw64 = None              # FP64 ||dw||_rms from the converged state
 
     @property
     def time_to_converge(self):
@@ -80,12 +84,53 @@ def device_memory(lib):
     return f.value, t.value
 
 
+def _jsonable(x):
+    """Request ids may be tuples (the ICA workflow's): JSON keeps them as
+    tagged lists."""
+    if isinstance(x, tuple):
+        return {"t": [_jsonable(v) for v in x]}
+    return x
+
+
+def _from_json(x):
+    if isinstance(x, dict) and "t" in x:
+        return tuple(_from_json(v) for v in x["t"])
+    return x
+
+
+def _write_rst(path, iteration, w, pf):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    pf = np.ascontiguousarray(pf, dtype=np.float64)
+    with open(path, "wb") as f:
+        f.write(b"SCFTRST1")
+        f.write(np.array([iteration, pf.size], dtype=np.uint64).tobytes())
+        f.write(w.tobytes())
+        f.write(pf.tobytes())
+
+
+def _read_rst(path):
+    raw = open(path, "rb").read()
+    if raw[:8] != b"SCFTRST1":
+        raise ValueError("%s is not a full-precision restart file" % path)
+    it, V = np.frombuffer(raw[8:24], dtype=np.uint64)
+    data = np.frombuffer(raw[24:], dtype=np.float64)
+    if data.size != 14 * int(V):
+        raise ValueError("%s: %d values, %d expected" % (path, data.size, 14 * int(V)))
+    return int(it), data[:13 * int(V)].reshape(13, int(V)).copy(), data[13 * int(V):].copy()
+
+
 class Pool:
     """One batch of slots sharing a chain structure."""
 
     def __init__(self, lib, proto, precision=abi.SCFT_FP32_SPLIT, max_slots=256, max_nx=0,
-                 hbm_fraction=0.85, tol=1e-5, max_iters=10 ** 6, record_every=0, diverge_dw=1e3):
+                 hbm_fraction=0.85, tol=1e-5, max_iters=10 ** 6, record_every=0, diverge_dw=1e3,
+                 fp64_check=False):
         self.lib = lib
+        # FP32-split pools: every converged state is also iterated once in
+        # FP64 (a one-slot FP64 batch), which gives each result its |dF|
+        # against FP64 and FP64's own ||dw|| at that state
+        self.fp64_check = bool(fp64_check) and precision != abi.SCFT_FP64
+        self._check = None
         self.proto = proto
         self.precision = precision
         self.tol = tol
@@ -94,6 +139,8 @@ class Pool:
         self.diverge_dw = diverge_dw
         self.per_slot = slot_bytes(lib, proto, max_nx, precision)
         free, total = device_memory(lib)
+        if self.fp64_check:
+            free -= slot_bytes(lib, proto, max_nx, abi.SCFT_FP64)
         fit = int(free * hbm_fraction // self.per_slot)
         self.nslots = max(1, min(max_slots, fit))
         self.batch = abi.Batch(lib, proto, self.nslots, max_nx=max_nx, precision=precision)
@@ -165,12 +212,35 @@ class Pool:
         req.done_t = now
         req.F = out.free_energy
         req.dw = out.dw_rms
-        if req.keep_fields and status == "converged":
-            req.fields = self.batch.get_fields(s)
+        if status == "converged" and (req.keep_fields or self.fp64_check):
+            fields = self.batch.get_fields(s)
+            if req.keep_fields:
+                req.fields = fields
+            if self.fp64_check:
+                self._fp64_eval(req, fields)
         self.batch.activate(s, False)
         self.free_slots.append(s)
         req.slot = None
         self.done.append(req)
+
+    def _fp64_eval(self, req, fields):
+        """One FP64 iteration from the converged fields: F64 (F of that state
+        in FP64 arithmetic throughout, the oracle's to 1e-10), |F - F64| and
+        the FP64 ||dw|| there."""
+        import configs
+        if self._check is None:
+            self._check = abi.Batch(self.lib, self.proto, 1, max_nx=self.batch.max_nx, precision=abi.SCFT_FP64)
+        c = self._check
+        c.init_slot(0, req.params)
+        c.set_fields(0, *fields)
+        if req.params.mask:
+            c.set_mask(0, req.mask if req.mask is not None else configs.brush_mask(req.params))
+        c.activate(0, True)
+        out = c.iterate([req.start_iter + req.iters])[0]
+        c.activate(0, False)
+        req.F64 = out.free_energy
+        req.dF = abs(req.F - out.free_energy)
+        req.dw64 = out.dw_rms
 
     # ---------------------------------------------------------- iteration
     def step(self):
@@ -219,6 +289,85 @@ class Pool:
                 self._retire(s, "max_iters", o, now)
         return len(self.running)
 
+    # ------------------------------------------------------- checkpoint
+    def checkpoint(self, directory):
+        """Save every running and queued request so that resume() continues
+        each exactly where it stopped (the iteration is deterministic and its
+        state is the fields, plus the noise stream's position): parameters,
+        iteration count and numbering, history, budget, noise-stream state,
+        and the fields at full precision in the drop-in driver's
+        restart_fields.bin layout ("SCFTRST1", u64 iteration, u64 V, then
+        w[13][V] and p[V] as FP64).  Call it between iterations."""
+        import json
+        import os
+        if self._inflight:
+            raise RuntimeError("checkpoint with an iteration in flight: step_complete first")
+        os.makedirs(directory, exist_ok=True)
+        entries = []
+
+        def put(k, req, fields):
+            e = {"rid": _jsonable(req.rid), "params": bytes(req.params).hex(), "iters": req.iters,
+                 "start_iter": req.start_iter, "max_iters": req.max_iters, "history": req.history,
+                 "keep_fields": req.keep_fields, "rng": list(req.rng) if req.rng is not None else None,
+                 "seed": None, "mask": None}
+            if fields is not None:
+                w, pf = fields
+                e["fields"] = "req%d.bin" % k
+                _write_rst(os.path.join(directory, e["fields"]), req.start_iter + req.iters - 1, w, pf)
+            else:
+                kind, arg = req.seed
+                if kind == SEED_FIELDS:
+                    e["fields"] = "req%d.bin" % k
+                    _write_rst(os.path.join(directory, e["fields"]), 0, arg[0], arg[1])
+                elif kind == SEED_FROM:
+                    e["seed"] = [kind, _jsonable(arg)]
+                else:
+                    e["seed"] = [kind, arg]
+            if req.mask is not None:
+                e["mask"] = "req%d_mask.npy" % k
+                np.save(os.path.join(directory, e["mask"]), np.asarray(req.mask))
+            entries.append(e)
+
+        k = 0
+        for s in sorted(self.running):
+            req = self.running[s]
+            if req.params.relaxation > 0 or req.rng is not None:
+                req.rng = self.batch.rng_state(s)
+            put(k, req, self.batch.get_fields(s))
+            k += 1
+        for req in self.queue:
+            put(k, req, None)
+            k += 1
+        with open(os.path.join(directory, "pool.json"), "w") as f:
+            json.dump({"format": 1, "requests": entries}, f)
+
+    def resume(self, directory):
+        """Queue the requests of a checkpoint() directory, running ones first
+        (each continues from its saved fields, iteration count and noise
+        state), in their saved order."""
+        import json
+        import os
+        with open(os.path.join(directory, "pool.json")) as f:
+            data = json.load(f)
+        out = []
+        for e in data["requests"]:
+            params = abi.ScftParams.from_buffer_copy(bytes.fromhex(e["params"]))
+            mask = np.load(os.path.join(directory, e["mask"])) if e.get("mask") else None
+            if e.get("fields"):
+                _, w, pf = _read_rst(os.path.join(directory, e["fields"]))
+                seed = (SEED_FIELDS, (w, pf))
+            else:
+                kind, arg = e["seed"]
+                seed = (kind, _from_json(arg) if kind == SEED_FROM else arg)
+            r = Request(_from_json(e["rid"]), params, seed, mask=mask, keep_fields=e["keep_fields"],
+                        rng=tuple(e["rng"]) if e["rng"] is not None else None, max_iters=e["max_iters"])
+            r.iters = e["iters"]
+            r.start_iter = e["start_iter"]
+            r.history = [tuple(h) for h in e["history"]]
+            self.submit(r)
+            out.append(r)
+        return out
+
     def take_done(self):
         """Return the finished requests and forget them (the pool keeps no
         reference, so their fields can be freed), except those a queued
@@ -254,6 +403,9 @@ class Pool:
 
     def close(self):
         self.batch.close()
+        if self._check is not None:
+            self._check.close()
+            self._check = None
 
 
 def request_from_run_dir(rid, directory, header="parameters.h", **kw):

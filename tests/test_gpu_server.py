@@ -92,18 +92,23 @@ def test_launch_complete_runs_two_contexts_at_once(abi, configs, lib):
 
 
 def _oracle_solve(orc, configs, p, seed, tol, max_iters):
-    """The oracle's solve of one request: iterations from 1 until ||dw|| < tol."""
+    """The oracle's solve of one request: iterations from 1 until ||dw|| <
+    tol (and one more); returns the iteration count and F, ||dw|| per
+    iteration."""
     st = orc.state(p)
     w, pf = configs.droplet_fields(p, radius=3.0, seed=seed)
     st.set_fields(w, pf)
     st.set_mask(configs.brush_mask(p))
-    for it in range(1, max_iters + 1):
+    F, dw, n = {}, {}, None
+    for it in range(1, max_iters + 2):
         o = st.iterate(it)
-        if o.dw_rms < tol:
+        F[it], dw[it] = o.free_energy, o.dw_rms
+        if n is None and o.dw_rms < tol:
+            n = it
+        elif n is not None:
             break
-    F = o.free_energy
     st.close()
-    return it, F
+    return n, F, dw
 
 
 def test_served_requests_admitted_mid_flight_match_oracle(abi, configs, orc, lib):
@@ -128,8 +133,87 @@ def test_served_requests_admitted_mid_flight_match_oracle(abi, configs, orc, lib
     admitted_mid = [r for r in reqs if r.admit_t > min(q.done_t for q in reqs)]
     assert len(admitted_mid) >= 3
     for r in reqs:
-        it, F = _oracle_solve(orc, configs, r.params, r.rid, 1e-5, 20000)
-        print("request", r.rid, r.status, r.iters, "F %.15g oracle %d %.15g" % (r.F, it, F), flush=True)
-        assert r.status == "converged" and r.iters == it
-        assert abs(r.F - F) <= 1e-9
+        it, F, dw = _oracle_solve(orc, configs, r.params, r.rid, 1e-5, 20000)
+        print("request", r.rid, r.status, r.iters, "F %.15g oracle %d %.15g" % (r.F, it, F[it]), flush=True)
+        assert r.status == "converged"
+        # the same iteration count, unless ||dw|| meets the tolerance within
+        # 0.1% there: ||dw|| is a difference of successive fields, so the
+        # ~1e-14 relative drift between the GPU and the oracle over
+        # thousands of iterations shows in it at the 1e-4 level
+        if r.iters != it:
+            edge = min(r.iters, it)
+            assert abs(r.iters - it) == 1 and abs(dw[edge] - 1e-5) <= 1e-8, (r.iters, it, dw[edge])
+        assert abs(r.F - F[r.iters]) <= 1e-9
+    pool.close()
+
+
+def test_checkpoint_resume_is_exact(abi, configs, lib, tmp_path):
+    """A pool checkpointed mid-run (two requests in flight, one with its
+    noise stream still on, a third still queued) and resumed in a new pool
+    finishes every request exactly as the uninterrupted pool does: the same
+    iteration counts and bit-identical free energies."""
+    import serve
+
+    def reqs():
+        out = []
+        for k, chi in enumerate((12.0, 14.0, 16.0)):
+            p = configs.chem_droplet(lx=12.0, lz=10.0, dr=1.0, ns=21, vol_dbk=0.2, vol_s1=0.05, len_s1=0.15,
+                                     g1=0.02, chi_ab=chi)
+            if k == 1:
+                p.relaxation = 60          # noise on for its first 60 iterations
+                p.dt_noise = 0.5
+            rng = (5, 6, 7) if k == 1 else None
+            out.append(serve.Request(("req", k), p, (serve.SEED_DROPLET, {"radius": 3.0, "seed": k}), rng=rng))
+        return out
+
+    def pool(r0):
+        return serve.Pool(lib, r0.params, precision=abi.SCFT_FP64, max_slots=2, max_nx=12, tol=1e-5,
+                          max_iters=20000)
+
+    a_reqs = reqs()
+    a = pool(a_reqs[0])
+    for r in a_reqs:
+        a.submit(r)
+    for _ in range(40):
+        a.step()
+    assert len(a.running) == 2 and len(a.queue) == 1
+    a.checkpoint(str(tmp_path / "ck"))
+    a.run()
+    want = {r.rid: (r.status, r.iters, r.F) for r in a.done}
+    a.close()
+    b = pool(a_reqs[0])
+    resumed = b.resume(str(tmp_path / "ck"))
+    assert [r.rid for r in resumed] == [("req", 0), ("req", 1), ("req", 2)]
+    b.run()
+    got = {r.rid: (r.status, r.iters, r.F) for r in b.done}
+    b.close()
+    print(want, got)
+    assert got == want
+
+
+def test_fp32_split_results_carry_fp64_check(abi, configs, orc, lib):
+    """An FP32-split pool with fp64_check: every converged request states
+    its F against an FP64 iteration of the same converged state (|dF| <=
+    1e-6 kT) and FP64's own ||dw|| there; the FP64 evaluation equals the
+    oracle's iteration of that state."""
+    import serve
+    reqs = []
+    for k, chi in enumerate((12.0, 15.0)):
+        p = configs.chem_droplet(lx=12.0, lz=10.0, dr=1.0, ns=21, vol_dbk=0.2, vol_s1=0.05, len_s1=0.15, g1=0.02,
+                                 chi_ab=chi)
+        reqs.append(serve.Request(k, p, (serve.SEED_DROPLET, {"radius": 3.0, "seed": k}), keep_fields=True))
+    pool = serve.Pool(lib, reqs[0].params, precision=abi.SCFT_FP32_SPLIT, max_slots=2, tol=1e-5, max_iters=20000,
+                      fp64_check=True)
+    for r in reqs:
+        pool.submit(r)
+    pool.run()
+    for r in reqs:
+        print("request", r.rid, r.status, r.iters, "F %.12f F64 %.12f dF %.2e dw64 %.2e" % (r.F, r.F64, r.dF, r.dw64))
+        assert r.status == "converged" and r.dF <= 1e-6 and r.dw64 < 2e-5
+        st = orc.state(r.params)
+        st.set_fields(*r.fields)
+        st.set_mask(configs.brush_mask(r.params))
+        o = st.iterate(r.iters + 1)
+        st.close()
+        assert abs(o.free_energy - r.F64) <= 1e-10 * max(1.0, abs(o.free_energy))
     pool.close()
