@@ -107,6 +107,14 @@ def c5_stream(lib, rate, duration, scale, precision, seed=0, drain=None):
     iters = sum(r.iters for r in done)
     pools = {("3D" if kk == key3 else "2D"): pl for kk, pl in server.pools.items()}
     occ = {n: (pl.slot_iterations / max(pl.iterations, 1)) for n, pl in pools.items()}
+    res = {"rate": rate, "duration": duration, "requests": len(reqs), "done": len(done), "done_3d": len(lat3),
+           "wall_s": wall, "req_per_s": len(done) / wall, "solve_iterations_per_s": iters / wall,
+           "p50_s": pc[50], "p99_s": pc[99], "p50_2d_s": pc2[50], "p99_2d_s": pc2[99], "p50_3d_s": pc3[50],
+           "p99_3d_s": pc3[99], "mean_active_2d": occ.get("2D", 0), "mean_active_3d": occ.get("3D", 0),
+           "slots_2d": pools["2D"].nslots if "2D" in pools else 0, "slots_3d": pools["3D"].nslots if "3D" in pools else 0,
+           "mean_budget_iterations": iters / max(len(done), 1), "scale": scale,
+           "full_length_solves_per_hour": len(done) / wall * scale * 3600.0,
+           "full_length_p50_s": (pc[50] or 0) / scale, "full_length_p99_s": (pc[99] or 0) / scale}
     print("C5 stream rate %.2f/s x %.0f s (%d requests, %d done, %d 3D): throughput %.3f req/s, %.0f solve-iterations/s; "
           "time-to-converge p50 %.2f s p99 %.2f s (2D p50 %.2f p99 %.2f, 3D p50 %.2f p99 %.2f); mean active 2D %.1f 3D %.1f "
           "of %d / %d slots; mean budget %.0f iterations (scale %g): at full length %.2f solves/hour, p50 %.0f s, p99 %.0f s"
@@ -116,6 +124,7 @@ def c5_stream(lib, rate, duration, scale, precision, seed=0, drain=None):
              iters / max(len(done), 1), scale, len(done) / wall * scale * 3600.0, (pc[50] or 0) / scale,
              (pc[99] or 0) / scale), flush=True)
     server.close()
+    return res
 
 
 def run(lib, name, protos, precision, iters):

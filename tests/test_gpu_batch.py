@@ -191,8 +191,8 @@ def test_periodic_axis_sizes_match_oracle(abi, configs, orc, lib, variant, nx, n
                                       (128, 6, 5), (256, 3, 5), (64, 3, 5), (64, 64, 5)])
 def test_periodic_register_ffts_fp32_split(abi, configs, orc, lib, nx, ny, nz):
     """The register FFTs with FP32 propagator slices (FP64 arithmetic)
-    against the FP64 oracle: F within 1e-5 of max(1, |F|), the updated
-    fields within 1e-5 of their largest value, over two iterations."""
+    against the FP64 oracle: F within 1e-6 kT, the updated fields within
+    1e-5 of their largest value, over two iterations."""
     os.environ["SCFT_XFFT"] = "reg"
     try:
         p = configs.chem_droplet(lx=nx * 0.25, ly=(ny * 0.25 if ny > 1 else 1.0), lz=nz * 0.25, dr=0.25, ns=11,
@@ -209,7 +209,7 @@ def test_periodic_register_ffts_fp32_split(abi, configs, orc, lib, nx, ny, nz):
         for it in (2, 3):
             a, o = ctx.iterate(it), ref.iterate(it)
             print((nx, ny, nz), it, "dF = %.3e" % abs(a.free_energy - o.free_energy), sched, flush=True)
-            assert _srel(a.free_energy, o.free_energy) <= 1e-5, (a.free_energy, o.free_energy, sched)
+            assert abs(a.free_energy - o.free_energy) <= 1e-6, (a.free_energy, o.free_energy, sched)
         wa, _ = ctx.get_fields()
         wb, _ = ref.get_fields()
         assert _rel(wa, wb) <= 1e-5

@@ -193,7 +193,7 @@ def test_fp32_split_close_to_fp64_oracle(abi, configs, orc, lib, name):
     for a, b in outs:
         dF = abs(a.free_energy - b.free_energy)
         print(name, "|dF| fp32-split vs fp64 oracle = %.3e" % dF)
-        assert dF <= 1e-5
+        assert dF <= 1e-6
     ctx.close()
 
 

@@ -3,7 +3,7 @@ transforms of a step in one workgroup per chain) against the oracle.
 
 - the C1 bulk melt and a bulk blend with four chain types (diblock q and
   q-dagger across the A/B junction, homopolymers A and C), FP64 to 1e-10
-  and FP32-split to 1e-5, with the z operator forced to the transform;
+  and FP32-split to 1e-6 kT in F (1e-5 in Q and the fields), with the z operator forced to the transform;
 - a batch of three slots with different chi through the fused step against
   the same batch with the step split into its z and x launches;
 - SCFT_PLANE=0 turns the fused step off; the split step then picks the
@@ -70,7 +70,10 @@ def test_plane_step_matches_oracle(abi, configs, orc, lib, z_fft, case, prec):
     for it in (2, 3, 4):
         a, o = ctx.iterate(it), ref.iterate(it)
         print(case, prec, it, "dF = %.3e" % abs(a.free_energy - o.free_energy), flush=True)
-        assert _srel(a.free_energy, o.free_energy) <= tol, (a.free_energy, o.free_energy)
+        if prec == "fp64":
+            assert _srel(a.free_energy, o.free_energy) <= tol, (a.free_energy, o.free_energy)
+        else:   # the FP32-split accuracy target: 1e-6 kT
+            assert abs(a.free_energy - o.free_energy) <= 1e-6, (a.free_energy, o.free_energy)
         for sp in range(17):
             assert _srel(a.qsum[sp], o.qsum[sp]) <= tol, (sp, a.qsum[sp], o.qsum[sp])
     assert _plane_count(ctx.schedule()) > 0, ctx.schedule()
