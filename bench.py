@@ -41,7 +41,7 @@ for _p in (ROOT, os.path.join(ROOT, "oracle"), PKG):
 
 METRIC = "converged SCFT solves/hour + p50/p99 time-to-converge at |\u0394F|\u22641e-6 kT vs FP64"
 PEAK_HBM_GBS = 8000.0
-CONV_PROFILE = os.path.join(ROOT, "profiles", "c2_convergence_%s.json")   # per precision
+CONV_PROFILE = os.path.join(ROOT, "profiles", "c2_convergence_%s.json")   # per precision (fp32, fp32_nearest, fp64)
 # HBM bytes per z-operator dispatch from the FETCH_SIZE / WRITE_SIZE runs
 # (calibrated with bin/copy_probe); see profiles/c2_served_b192_hbm_traffic.md
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c2_served_b192_hbm_traffic.json")
@@ -84,7 +84,7 @@ def time_iterations(pool, warmup, steps):
 
 
 def conv_stats(prec_name):
-    path = CONV_PROFILE % prec_name
+    path = CONV_PROFILE % prec_name.replace("-", "_")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -98,7 +98,7 @@ def conv_stats(prec_name):
     it = np.asarray(list(runs["iters"]) + cens, float)
     return {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)), "p99": float(np.percentile(it, 99)),
             "n": int(len(it)), "n_censored": len(cens), "censored_at": (min(cens) if cens else None),
-            "source": "profiles/c2_convergence_%s.json" % prec_name}
+            "source": "profiles/c2_convergence_%s.json" % prec_name.replace("-", "_")}
 
 
 def iteration_bytes(abi, lib, params, b_el):
@@ -210,7 +210,7 @@ def converge_run(abi, configs, serve, lib, seeds, prec_name, tlimit, out, fields
     there (the iteration is deterministic, so a resumed solve continues
     exactly where it stopped)."""
     import numpy as np
-    precision = abi.SCFT_FP32_SPLIT if prec_name == "fp32" else abi.SCFT_FP64
+    precision = {"fp32": abi.SCFT_FP32_SPLIT, "fp32-nearest": abi.SCFT_FP32_NEAREST}.get(prec_name, abi.SCFT_FP64)
     p = configs.params_c2()
     prior = []
     reqs = []
@@ -374,7 +374,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200, help="timed batched iterations")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--slots", type=int, default=192, help="resident solves (capped by HBM footprint)")
-    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "fp32-nearest", "fp64"], default="fp32",
+                    help="fp32: FP32-split (slices rounded without bias); fp32-nearest: slices rounded to nearest")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline: seconds per oracle process")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline processes (default: host cores, <= 16)")
@@ -423,12 +424,12 @@ def main():
     if args.converge:
         seeds = converge_seeds(args.converge_seeds) if args.converge_seeds else list(range(args.converge))
         converge_run(abi, configs, serve, lib, seeds, args.precision, args.converge_seconds,
-                     args.converge_out or CONV_PROFILE % args.precision, args.converge_fields,
+                     args.converge_out or CONV_PROFILE % args.precision.replace("-", "_"), args.converge_fields,
                      args.converge_state_in, args.converge_state_out, args.converge_perturb,
                      converge_seeds(args.converge_perturb_seeds) if args.converge_perturb else ())
         return
 
-    precision = abi.SCFT_FP32_SPLIT if args.precision == "fp32" else abi.SCFT_FP64
+    precision = {"fp32": abi.SCFT_FP32_SPLIT, "fp32-nearest": abi.SCFT_FP32_NEAREST}.get(args.precision, abi.SCFT_FP64)
     log("served pool: C2, %s, up to %d slots" % (args.precision, args.slots))
     pool = make_pool(abi, configs, serve, lib, args.slots, precision)
     B = pool.nslots
@@ -471,7 +472,7 @@ def main():
     # E and K are per-block operands and not counted) times the voxel-steps one
     # z launch processes, over the z launch's duration.  The kernel itself also
     # reads E (8 B per voxel-step); its bytes are reported beside.
-    b_el = 4.0 if args.precision == "fp32" else 8.0
+    b_el = 8.0 if args.precision == "fp64" else 4.0
     units = zt["voxel_steps"]
     alg_bytes = 2.0 * b_el * units
     ach = alg_bytes / (zt["avg_us"] * 1e-6) / 1e9 if zt["avg_us"] > 0 else None
@@ -495,7 +496,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2D thin-film nanodroplet served solves (C2): 256x1x128, NS 101, MASK 1, "
                                "dbk AB f=0.5 vol 0.15 + homo C 0.85 + brush S1 0.05, chiN AB 25 AC=BC 40, G1 0.01",
-                   "slots": B, "precision": args.precision + ("-split" if args.precision == "fp32" else ""),
+                   "slots": B, "precision": {"fp32": "fp32-split (FP32 slices, unbiased rounding)",
+                                             "fp32-nearest": "fp32-split, slices rounded to nearest"}.get(args.precision,
+                                                                                                           "fp64"),
                    "measured": {"instance_iterations_per_s": inst_iter_per_s,
                                 "ms_per_batched_iteration": ms_per_step,
                                 "schedule": sched},

@@ -38,8 +38,12 @@ enum {
 };
 
 enum {
-    SCFT_FP64       = 0,    /* everything in double precision */
-    SCFT_FP32_SPLIT = 1     /* FP32 propagators/FFTs/contour sums (Kahan), FP64 Q, F, update */
+    SCFT_FP64         = 0,  /* everything in double precision */
+    SCFT_FP32_SPLIT   = 1,  /* FP32 propagator slices (stored with unbiased rounding), FP64 transforms,
+                               Kahan-compensated FP64 contour sums, FP64 Q, F, update */
+    SCFT_FP32_NEAREST = 2   /* as SCFT_FP32_SPLIT, slices rounded to nearest.  Rounding to nearest
+                               erases perturbations below an FP32 step that FP64 carries, so an
+                               unstable symmetric state that FP64 leaves can look stable */
 };
 
 /*

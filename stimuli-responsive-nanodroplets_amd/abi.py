@@ -15,7 +15,8 @@ FIELD_NAMES = list("ABCDEFGHIJ") + ["S1", "S2", "S3"]
 SPECIES_NAMES = ["dbk1", "dbk2", "dbk3", "dbk4"] + ["homo" + c for c in "ABCDEFGHIJ"] + ["S1", "S2", "S3"]
 
 SCFT_FP64 = 0
-SCFT_FP32_SPLIT = 1
+SCFT_FP32_SPLIT = 1      # FP32 slices, unbiased rounding
+SCFT_FP32_NEAREST = 2    # FP32 slices, round to nearest
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "out of device memory",
           -4: "volumes do not sum to 1", -5: "unsupported size", -6: "non-finite value",

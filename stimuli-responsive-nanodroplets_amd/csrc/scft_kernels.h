@@ -106,6 +106,41 @@ __device__ __forceinline__ void stg_pair(T *p, unsigned i, T a, T b)
     *(__attribute__((address_space(1))) v2 *)((char *)p + i * (unsigned)sizeof(T)) = v2{a, b};
 }
 
+// A propagator value stored in the slice type T.  FP64 slices keep the
+// value; FP32 slices round it without bias when the op carries a rounding
+// key (rkey != 0): to one of its two FP32 neighbours with probability set by
+// the distance, by adding a hashed 29-bit integer (from the op's key and
+// the element offset) to the FP64 bits and truncating toward zero.
+// Round-to-nearest maps mirror-image values that differ by less than half
+// an FP32 step to the same value, every step, so an asymmetric perturbation
+// below that size is erased rather than carried; FP64 keeps it, and a
+// symmetric state that is unstable in FP64 then looks stable in FP32.
+// Unbiased rounding keeps each value right on average and leaves such a
+// perturbation a chance to grow.  Deterministic: the same chain, step and
+// element always draw the same bits.
+template <typename T>
+__device__ __forceinline__ T store_round(double v, unsigned rkey, unsigned i);
+
+template <>
+__device__ __forceinline__ double store_round<double>(double v, unsigned, unsigned)
+{
+    return v;
+}
+
+template <>
+__device__ __forceinline__ float store_round<float>(double v, unsigned rkey, unsigned i)
+{
+    if (!rkey) return float(v);
+    unsigned h = rkey ^ (i * 0x9E3779B1u);
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v) + (unsigned long long)(h >> 3);
+    return __double2float_rz(__longlong_as_double((long long)b));
+}
+
 // ------------------------------------------------------------ FFT line op
 
 struct LineOp {
@@ -123,6 +158,7 @@ struct LineOp {
     int pairs;           // lines 2i, 2i+1 are adjacent and 2-element aligned (nlines even)
     int npass;
     int radix[kMaxPass];
+    unsigned rkey;       // unbiased-rounding key of FP32 stores (store_round), 0: round to nearest
 };
 
 __device__ __forceinline__ int line_base(const LineOp &op, int l)
@@ -206,7 +242,7 @@ __global__ void __launch_bounds__(kThreads) k_lineop(const LineOp *__restrict__ 
         const int a = line_base(op, g0 + r) + t * op.stride;
         T v = *slot * scale;
         if (op.epost) v *= op.epost[a];
-        dst[a] = Tout(v);
+        dst[a] = store_round<Tout>(v, op.rkey, unsigned(a));
     }
 }
 
@@ -217,6 +253,7 @@ struct DenseOp {
     void *dst;
     const double *epre, *epre2, *epost;
     const float *e2f;    // E^2 of epre's field in FP32 (inside a block, FP32-split), or null
+    unsigned rkey;       // unbiased-rounding key of FP32 stores (store_round), 0: round to nearest
     const double *MsT;   // [hs][ldm]: MsT[j][k] = Ms[k][j]
     const double *MaT;   // [ha][ldm]
     int n, hs, ha, ldm;
@@ -298,11 +335,11 @@ __global__ void __launch_bounds__(kThreads) k_zdense(const DenseOp *__restrict__
                 if (k >= hs) continue;
                 double y1 = u[i][c] + v[i][c];
                 if (op.epost) y1 *= op.epost[base + k];
-                dst[base + k] = Tout(y1);
+                dst[base + k] = store_round<Tout>(y1, op.rkey, unsigned(base + k));
                 if (k < ha) {
                     double y2 = u[i][c] - v[i][c];
                     if (op.epost) y2 *= op.epost[base + n - 1 - k];
-                    dst[base + n - 1 - k] = Tout(y2);
+                    dst[base + n - 1 - k] = store_round<Tout>(y2, op.rkey, unsigned(base + n - 1 - k));
                 }
             }
         }
@@ -516,11 +553,11 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *
                 const double u = U[kt][r], v = W[kt][r];
                 double y = u + v;
                 if (op.epost) y *= ldg(op.epost, row + k);
-                stg(dst, row + k, Tout(y));
+                stg(dst, row + k, store_round<Tout>(y, op.rkey, row + k));
                 if (k < ha) {
                     double y2 = u - v;
                     if (op.epost) y2 *= ldg(op.epost, row + n - 1 - k);
-                    stg(dst, row + n - 1 - k, Tout(y2));
+                    stg(dst, row + n - 1 - k, store_round<Tout>(y2, op.rkey, row + n - 1 - k));
                 }
             }
         }
@@ -656,8 +693,8 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
             if (rr >= nr) continue;
             const unsigned row = unsigned(rbase + rr) * unsigned(n);
             const double u = U[kt][r], v = W[kt][r];
-            stg(dst, row + k, Tout(u + v));
-            stg(dst, row + n - 1 - k, Tout(u - v));
+            stg(dst, row + k, store_round<Tout>(u + v, op.rkey, row + k));
+            stg(dst, row + n - 1 - k, store_round<Tout>(u - v, op.rkey, row + n - 1 - k));
         }
     }
 }
@@ -897,7 +934,7 @@ __global__ void __launch_bounds__(64 * kFftWaves) k_fftwave(const LineOp *__rest
         const int a = lbase[r] + t * stride;
         double v = reinterpret_cast<const double *>(&buf[(r >> 1) * LS + t])[r & 1] * scale;
         if (op.epost) v *= ldg(op.epost, a);
-        stg(dst, a, Tout(v));
+        stg(dst, a, store_round<Tout>(v, op.rkey, unsigned(a)));
     }
 }
 
@@ -1101,11 +1138,12 @@ __device__ __forceinline__ void freg_apply(const LineOp &op, int r0, cpx<double>
             y0 *= ldg(op.epost, o0 + m * step);
             y1 *= ldg(op.epost, o1 + m * step);
         }
+        const unsigned a0 = o0 + m * step, a1 = o1 + m * step;
         if (op.pairs) {
-            if (ok0) stg_pair(dst, o0 + m * step, Tout(y0), Tout(y1));
+            if (ok0) stg_pair(dst, a0, store_round<Tout>(y0, op.rkey, a0), store_round<Tout>(y1, op.rkey, a0 + 1));
         } else {
-            if (ok0) stg(dst, o0 + m * step, Tout(y0));
-            if (ok1) stg(dst, o1 + m * step, Tout(y1));
+            if (ok0) stg(dst, a0, store_round<Tout>(y0, op.rkey, a0));
+            if (ok1) stg(dst, a1, store_round<Tout>(y1, op.rkey, a1));
         }
     }
 }
@@ -1312,11 +1350,12 @@ __device__ __forceinline__ void freg2_apply(const LineOp &op, int r0, cpx<double
             y0 *= ldg(op.epost, o0 + m * step);
             y1 *= ldg(op.epost, o1 + m * step);
         }
+        const unsigned a0 = o0 + m * step, a1 = o1 + m * step;
         if (op.pairs) {
-            if (ok0) stg_pair(dst, o0 + m * step, Tout(y0), Tout(y1));
+            if (ok0) stg_pair(dst, a0, store_round<Tout>(y0, op.rkey, a0), store_round<Tout>(y1, op.rkey, a0 + 1));
         } else {
-            if (ok0) stg(dst, o0 + m * step, Tout(y0));
-            if (ok1) stg(dst, o1 + m * step, Tout(y1));
+            if (ok0) stg(dst, a0, store_round<Tout>(y0, op.rkey, a0));
+            if (ok1) stg(dst, a1, store_round<Tout>(y1, op.rkey, a1));
         }
     }
 }
@@ -1369,6 +1408,7 @@ struct PlaneOp {
     const double *epre2;       // factor the stored slice owes (null, epre, or another E)
     const double *Kz, *Kx;     // 64-point spectra, natural order
     const double *twz, *twx;   // w_64^{-e}, complex interleaved
+    unsigned rkey;             // unbiased-rounding key of FP32 stores (store_round)
 };
 
 constexpr int kPlaneN = 64;        // points per axis
@@ -1480,7 +1520,8 @@ __global__ void __launch_bounds__(64 * kPlaneWaves) k_plane64(const PlaneOp *__r
         }
         __syncthreads();
         Tout *dst = static_cast<Tout *>(op.dst);
-        for (int i = tid; i < N * N; i += NT) stg(dst, unsigned(i), Tout(Pl[(i >> 6) * PS + (i & 63)]));
+        for (int i = tid; i < N * N; i += NT)
+            stg(dst, unsigned(i), store_round<Tout>(Pl[(i >> 6) * PS + (i & 63)], op.rkey, unsigned(i)));
         __syncthreads();   // the next op overwrites the plane
     }
 }

@@ -11,7 +11,8 @@
  * by a thread of their own while the GPU runs on.
  *
  * Replaces SCFT.c:82-1664.  Environment: SCFT_PRECISION=fp32 selects the
- * FP32-split propagators (default FP64, the reference's arithmetic).
+ * FP32-split propagators (fp32-nearest: slices rounded to nearest; default
+ * FP64, the reference's arithmetic).
  * SCFT_EXACT_RESTART=1 also writes restart_fields.bin beside every
  * restart_fields.dat: the fields in full precision (the text file keeps the
  * reference's 6 decimals, SCFT.c:1645-1646).  A restart (counter != 0)
@@ -312,7 +313,10 @@ int main(void)
 
     const char *prec = getenv("SCFT_PRECISION");
     scft_ctx *ctx;
-    check(scft_create(&p, (prec && strcmp(prec, "fp32") == 0) ? SCFT_FP32_SPLIT : SCFT_FP64, &ctx), "scft_create");
+    const int precision = !prec ? SCFT_FP64
+                          : strcmp(prec, "fp32") == 0 ? SCFT_FP32_SPLIT
+                          : strcmp(prec, "fp32-nearest") == 0 ? SCFT_FP32_NEAREST : SCFT_FP64;
+    check(scft_create(&p, precision, &ctx), "scft_create");
 
     double *w = calloc(13 * V, sizeof(double)), *pf = calloc(V, sizeof(double));
     if (!w || !pf) die("calloc");

@@ -43,6 +43,14 @@ A replacing trial then enters with its FP64 F and fields.  The FP32-split
 converged states are FP64 fixed points at the tolerance, so polishing
 takes few iterations.
 
+Precision.  In FP32-split runs the rounding of the FP32 slices matters at
+saddles: FP64 carries the unstable mode of a symmetric state at its own
+rounding level (2^-53), slices rounded to nearest (abi.SCFT_FP32_NEAREST)
+do not carry it at all, and unbiased rounding (abi.SCFT_FP32_SPLIT) at
+2^-24.  On small films, where FP64 stops some solves at saddles before it
+leaves them, SCFT_FP32_NEAREST reproduces FP64's decisions and
+SCFT_FP32_SPLIT leaves those saddles and decides differently.
+
 Neighbours: in-plane (i-1, j), (i+1, j), (i, j-1), (i, j+1) and cross-plane
 the same (i, j) on the previous and next plane of the cycle P0 -> P1 -> P2
 -> P0 (README.md:76-79).
