@@ -243,6 +243,26 @@ int scft_time_launches(scft_ctx *ctx, int kind, int reps, double *avg_us, double
 /* Human-readable schedule summary (z operator kind, launches per iteration). */
 int scft_get_schedule(const scft_ctx *ctx, char *buf, int len);
 
+/*
+ * scft_step_noalloc: one contour step of one chain, q_out = E M(E q_in)
+ * with E = exp(-DS w/2), on device arrays the caller owns (V doubles each,
+ * idx = k + NZ*(j + NY*i)): w_dev the chain's field, q_in_dev the slice s,
+ * q_out_dev the slice s + 1.  The same step as scft_propagate takes per
+ * slice (M the periodic or, for MASK 1, Dirichlet z transform and the
+ * periodic x and y transforms), without a context.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  It allocates nothing: its
+ * tables, E and the scratch between axis operators live in one static
+ * device array per GPU (about 4.1 MB, held until the process ends).
+ * Limits: V <= 262144 voxels and transform lengths (NX, NY, and NZ, or
+ * 2(NZ+1) for MASK 1) <= 1024, else SCFT_EUNSUPPORTED.  Calls on one GPU
+ * share that array: calls that overlap on one GPU race, so callers on
+ * several streams must order them.  stream: the hipStream_t to launch on
+ * (NULL for the null stream); the call returns once the work is queued.
+ */
+int scft_step_noalloc(const scft_params *p, const double *w_dev, const double *q_in_dev, double *q_out_dev,
+                      void *stream);
+
 /* Wait for the context's stream. */
 int scft_sync(scft_ctx *ctx);
 /* Last error text of this thread (static storage). */

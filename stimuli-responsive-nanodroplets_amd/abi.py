@@ -177,6 +177,7 @@ def load_library(path=None):
         "scft_slot_bytes": ([P(ScftParams), C.c_int, C.c_int, P(C.c_size_t)], C.c_int),
         "scft_device_memory": ([P(C.c_size_t), P(C.c_size_t)], C.c_int),
         "scft_time_launches": ([vp, C.c_int, C.c_int, dp, dp, dp, dp, P(C.c_int)], C.c_int),
+        "scft_step_noalloc": ([P(ScftParams), vp, vp, vp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1984,3 +1984,165 @@ int scft_get_schedule(const scft_ctx *c, char *buf, int len)
 
 }  // extern "C"
 
+// ============================================== one contour step, no allocation
+//
+// scft_step_noalloc: one contour step of one chain on caller device arrays,
+// for callers that cannot allocate device memory.  Its tables, its Boltzmann
+// factor, its transform-op blocks and the scratch between axis operators
+// live in `scratch`, a file-scope device array that every GPU running the
+// library holds until the process ends.  Host code never uses the array's
+// name as an address: the per-device address comes from
+// hipGetSymbolAddress, resolved once per device with that device current.
+
+namespace {
+
+constexpr size_t kNaMaxV = size_t(1) << 18;          // voxels per call
+constexpr int kNaMaxL = 1024;                         // transform length per axis
+constexpr size_t kNaOpsBytes = 1024;                  // 3 LineOp blocks
+constexpr size_t kNaTableBytes = size_t(3) * kNaMaxL * (sizeof(double) + sizeof(cpx<double>));
+constexpr size_t kNaScratchBytes = kNaOpsBytes + kNaTableBytes + 2 * kNaMaxV * sizeof(double);
+static_assert(3 * sizeof(LineOp) <= kNaOpsBytes, "op blocks exceed their region");
+static_assert(kNaOpsBytes % 256 == 0 && kNaTableBytes % 256 == 0, "regions stay 256-byte aligned");
+
+std::mutex g_na_mu;
+void *g_na_ptr[64] = {};                              // scratch address per device
+bool g_na_attr = false;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+namespace scft {
+
+// K and the twiddles of one axis (the tables get_fft computes on the host).
+__global__ void __launch_bounds__(kThreads) k_na_tables(double *K, cpx<double> *tw, int L, double alpha)
+{
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < L; m += gridDim.x * blockDim.x) {
+        const double km = 2.0 * M_PI * (L / 2.0 - fabs(m - L / 2.0)) / L;
+        K[m] = exp(-alpha * km * km);
+        const double a = 2.0 * M_PI * double(m) / double(L);
+        tw[m] = {cos(a), -sin(a)};
+    }
+}
+
+// E = exp(-DS w / 2) of the chain's field (k_prep's formula).
+__global__ void __launch_bounds__(kThreads) k_na_factor(const double *w, double *E, double ds, int V)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x)
+        E[i] = exp(ds * (-w[i] / 2.0));
+}
+
+// Writes a transform-op block (passed by value) into device memory.
+__global__ void k_na_put_op(LineOp op, LineOp *dst) { *dst = op; }
+
+}  // namespace scft
+
+namespace {
+
+int na_scratch(unsigned char **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_err("device index out of range");
+        return SCFT_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_attr) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_lineop<double, double>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        g_na_attr = true;
+    }
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return SCFT_EHIP;
+        }
+        g_na_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char *>(g_na_ptr[dev]);
+    return SCFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scft_step_noalloc(const scft_params *p, const double *w_dev, const double *q_in_dev, double *q_out_dev,
+                      void *stream)
+{
+    if (!p || !w_dev || !q_in_dev || !q_out_dev) return SCFT_EINVAL;
+    scft_derived d;
+    RETCHK(scft_derive(p, &d));
+    const size_t V = size_t(d.nx) * d.ny * d.nz;
+    if (V > kNaMaxV) {
+        set_err("scft_step_noalloc: " + std::to_string(V) + " voxels exceed the limit of " + std::to_string(kNaMaxV));
+        return SCFT_EUNSUPPORTED;
+    }
+    const int dir = p->mask ? 1 : 0;
+    const double DS = 1.0 / (p->ns - 1);
+    const double DX = p->lx / d.nx, DY = p->ly / d.ny, DZ = p->lz / d.nz;
+    struct Axis { int axis, L; double alpha; Radices rad; };
+    std::vector<Axis> axes;
+    axes.push_back({2, dir ? 2 * (d.nz + 1) : d.nz, DS / (DZ * DZ), {}});
+    if (d.ny > 1) axes.push_back({1, d.ny, DS / (DY * DY), {}});
+    if (d.nx > 1) axes.push_back({0, d.nx, DS / (DX * DX), {}});
+    for (auto &a : axes) {
+        if (a.L > kNaMaxL || !plan_radices(a.L, a.rad)) {
+            set_err("scft_step_noalloc: transform length " + std::to_string(a.L) + " is not supported");
+            return SCFT_EUNSUPPORTED;
+        }
+    }
+    unsigned char *base = nullptr;
+    RETCHK(na_scratch(&base));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    LineOp *ops = reinterpret_cast<LineOp *>(base);
+    unsigned char *tables = base + kNaOpsBytes;
+    double *E = reinterpret_cast<double *>(base + kNaOpsBytes + kNaTableBytes);
+    double *tmp = E + kNaMaxV;
+    hipLaunchKernelGGL(k_na_factor, dim3(grid_for(V, 512)), dim3(kThreads), 0, st, w_dev, E, DS, int(V));
+    const int nax = int(axes.size());
+    for (int ai = 0; ai < nax; ++ai) {
+        const Axis &a = axes[ai];
+        double *K = reinterpret_cast<double *>(tables + size_t(ai) * kNaMaxL * (sizeof(double) + sizeof(cpx<double>)));
+        cpx<double> *tw = reinterpret_cast<cpx<double> *>(K + kNaMaxL);
+        hipLaunchKernelGGL(k_na_tables, dim3(grid_for(a.L)), dim3(kThreads), 0, st, K, tw, a.L, a.alpha);
+        LineOp op{};
+        op.src = ai == 0 ? static_cast<const void *>(q_in_dev) : static_cast<const void *>(tmp);
+        op.dst = ai == nax - 1 ? static_cast<void *>(q_out_dev) : static_cast<void *>(tmp);
+        op.epre = ai == 0 ? E : nullptr;
+        op.epre2 = nullptr;
+        op.epost = ai == nax - 1 ? E : nullptr;
+        op.K = K;
+        op.tw = tw;
+        op.L = a.L;
+        op.npass = a.rad.npass;
+        for (int i = 0; i < kMaxPass; ++i) op.radix[i] = i < a.rad.npass ? a.rad.r[i] : 1;
+        if (a.axis == 2) {
+            op.n = d.nz;
+            op.dirichlet = dir;
+            op.nlines = d.nx * d.ny;
+            op.c1 = 1; op.s0 = 0; op.s1 = d.nz; op.stride = 1;
+        } else if (a.axis == 1) {
+            op.n = d.ny;
+            op.nlines = d.nx * d.nz;
+            op.c1 = d.nz; op.s0 = 1; op.s1 = d.nz * d.ny; op.stride = d.nz;
+        } else {
+            op.n = d.nx;
+            op.nlines = d.ny * d.nz;
+            op.c1 = d.ny * d.nz; op.s0 = 1; op.s1 = 0; op.stride = d.ny * d.nz;
+        }
+        op.pairs = (op.s0 == 1 && op.c1 % 2 == 0 && op.s1 % 2 == 0 && op.stride % 2 == 0) ? 1 : 0;
+        hipLaunchKernelGGL(k_na_put_op, dim3(1), dim3(1), 0, st, op, ops + ai);
+        const int P = lines_P(a.L);
+        const size_t lds = size_t(2) * P * a.L * sizeof(cpx<double>);
+        hipLaunchKernelGGL((k_lineop<double, double>), dim3((op.nlines + 2 * P - 1) / (2 * P), 1), dim3(kThreads), lds,
+                           st, static_cast<const LineOp *>(ops + ai), P);
+    }
+    HIPCHK(hipGetLastError());
+    return SCFT_OK;
+}
+
+}  // extern "C"
