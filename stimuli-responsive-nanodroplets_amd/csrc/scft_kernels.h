@@ -106,18 +106,20 @@ __device__ __forceinline__ void stg_pair(T *p, unsigned i, T a, T b)
     *(__attribute__((address_space(1))) v2 *)((char *)p + i * (unsigned)sizeof(T)) = v2{a, b};
 }
 
-// A propagator value stored in the slice type T.  FP64 slices keep the
-// value; FP32 slices round it without bias when the op carries a rounding
-// key (rkey != 0): to one of its two FP32 neighbours with probability set by
-// the distance, by adding a hashed 29-bit integer (from the op's key and
-// the element offset) to the FP64 bits and truncating toward zero.
-// Round-to-nearest maps mirror-image values that differ by less than half
-// an FP32 step to the same value, every step, so an asymmetric perturbation
-// below that size is erased rather than carried; FP64 keeps it, and a
-// symmetric state that is unstable in FP64 then looks stable in FP32.
-// Unbiased rounding keeps each value right on average and leaves such a
-// perturbation a chance to grow.  Deterministic: the same chain, step and
-// element always draw the same bits.
+// A propagator slice value stored in the slice type T.  FP64 slices keep
+// the value; FP32 slices round it without bias when the op carries a
+// rounding key (rkey != 0): a hashed 29-bit integer (from the op's key and
+// the element offset) is added to the FP64 bits and the 29 bits below an
+// FP32 mantissa are cleared, so the value goes to one of its two FP32
+// neighbours with probability set by the distance.  Round-to-nearest maps
+// mirror-image values that differ by less than half an FP32 step to the
+// same value at every step, so an asymmetric perturbation below that size
+// is erased rather than carried; FP64 keeps it, and a symmetric state that
+// is unstable in FP64 then looks stable in FP32.  Unbiased rounding keeps
+// each value right on average and leaves such a perturbation a chance to
+// grow.  Deterministic: the same chain, step and element always draw the
+// same bits.  The last axis of a step (the stored slice) rounds this way;
+// the scratch between axes rounds to nearest.
 template <typename T>
 __device__ __forceinline__ T store_round(double v, unsigned rkey, unsigned i);
 
@@ -131,14 +133,12 @@ template <>
 __device__ __forceinline__ float store_round<float>(double v, unsigned rkey, unsigned i)
 {
     if (!rkey) return float(v);
-    unsigned h = rkey ^ (i * 0x9E3779B1u);
-    h ^= h >> 16;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v) + (unsigned long long)(h >> 3);
-    return __double2float_rz(__longlong_as_double((long long)b));
+    unsigned h = (i ^ rkey) * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    const unsigned long long b =
+        ((unsigned long long)__double_as_longlong(v) + (unsigned long long)(h >> 3)) & ~0x1FFFFFFFull;
+    return float(__longlong_as_double((long long)b));   // exact for normal FP32 values
 }
 
 // ------------------------------------------------------------ FFT line op
@@ -253,7 +253,6 @@ struct DenseOp {
     void *dst;
     const double *epre, *epre2, *epost;
     const float *e2f;    // E^2 of epre's field in FP32 (inside a block, FP32-split), or null
-    unsigned rkey;       // unbiased-rounding key of FP32 stores (store_round), 0: round to nearest
     const double *MsT;   // [hs][ldm]: MsT[j][k] = Ms[k][j]
     const double *MaT;   // [ha][ldm]
     int n, hs, ha, ldm;
@@ -335,11 +334,11 @@ __global__ void __launch_bounds__(kThreads) k_zdense(const DenseOp *__restrict__
                 if (k >= hs) continue;
                 double y1 = u[i][c] + v[i][c];
                 if (op.epost) y1 *= op.epost[base + k];
-                dst[base + k] = store_round<Tout>(y1, op.rkey, unsigned(base + k));
+                dst[base + k] = Tout(y1);
                 if (k < ha) {
                     double y2 = u[i][c] - v[i][c];
                     if (op.epost) y2 *= op.epost[base + n - 1 - k];
-                    dst[base + n - 1 - k] = store_round<Tout>(y2, op.rkey, unsigned(base + n - 1 - k));
+                    dst[base + n - 1 - k] = Tout(y2);
                 }
             }
         }
@@ -553,11 +552,11 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_zdense_mfma(const DenseOp *
                 const double u = U[kt][r], v = W[kt][r];
                 double y = u + v;
                 if (op.epost) y *= ldg(op.epost, row + k);
-                stg(dst, row + k, store_round<Tout>(y, op.rkey, row + k));
+                stg(dst, row + k, Tout(y));
                 if (k < ha) {
                     double y2 = u - v;
                     if (op.epost) y2 *= ldg(op.epost, row + n - 1 - k);
-                    stg(dst, row + n - 1 - k, store_round<Tout>(y2, op.rkey, row + n - 1 - k));
+                    stg(dst, row + n - 1 - k, Tout(y2));
                 }
             }
         }
@@ -693,8 +692,8 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
             if (rr >= nr) continue;
             const unsigned row = unsigned(rbase + rr) * unsigned(n);
             const double u = U[kt][r], v = W[kt][r];
-            stg(dst, row + k, store_round<Tout>(u + v, op.rkey, row + k));
-            stg(dst, row + n - 1 - k, store_round<Tout>(u - v, op.rkey, row + n - 1 - k));
+            stg(dst, row + k, Tout(u + v));
+            stg(dst, row + n - 1 - k, Tout(u - v));
         }
     }
 }

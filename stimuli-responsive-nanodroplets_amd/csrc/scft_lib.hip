@@ -694,12 +694,13 @@ void drop_graph(scft_ctx *c)
     c->graph = nullptr;
 }
 
-// Key of the unbiased rounding of a launch's FP32 stores: chain ci of its
-// slot, contour step t, axis position ai (the same for a slot wherever it
-// sits in a batch); 0 when the context rounds to nearest.
-unsigned round_key(const scft_ctx *c, int ci, int t, int ai)
+// Key of the unbiased rounding of the FP32 slice a launch stores: chain ci
+// of its slot, contour step t (the same for a slot wherever it sits in a
+// batch); 0 for launches that store the scratch between axes, and when the
+// context rounds to nearest.
+unsigned round_key(const scft_ctx *c, int ci, int t, int ai, bool last)
 {
-    if (!c->sround || c->tsize != 4) return 0;
+    if (!c->sround || c->tsize != 4 || !last) return 0;
     const unsigned k = (0x9E3779B1u * unsigned(ci + 1)) ^ (0x85EBCA6Bu * unsigned(t + 1)) ^ (0xC2B2AE35u * unsigned(ai + 1));
     return k ? k : 1u;
 }
@@ -736,7 +737,7 @@ int build_schedule(scft_ctx *c)
                     const Chain &ch = s.chains[ci];
                     if (ch.nsteps <= t) continue;
                     PlaneOp op{};
-                    op.rkey = round_key(c, int(ci), t, 0);
+                    op.rkey = round_key(c, int(ci), t, 0, true);
                     op.src = static_cast<const char *>(ch.base) + (size_t)t * V * ts;
                     op.dst = static_cast<char *>(ch.base) + (size_t)(t + 1) * V * ts;
                     op.epre = s.E + (size_t)ch.step_field(t) * V;
@@ -775,7 +776,7 @@ int build_schedule(scft_ctx *c)
                 for (size_t ci = 0; ci < s.chains.size(); ++ci) {
                     const Chain &ch = s.chains[ci];
                     if (ch.nsteps <= t) continue;
-                    const unsigned rkey = round_key(c, int(ci), t, ai);
+                    const unsigned rkey = round_key(c, int(ci), t, ai, ai == nax - 1);
                     const void *src = (ai == 0) ? static_cast<const char *>(ch.base) + (size_t)t * V * ts : (const void *)ch.tmp;
                     void *dst = (ai == nax - 1) ? static_cast<char *>(ch.base) + (size_t)(t + 1) * V * ts : (void *)ch.tmp;
                     // first axis: E of this step times the factor the stored slice owes
@@ -790,7 +791,6 @@ int build_schedule(scft_ctx *c)
                         op.epost = nullptr;
                         // inside a block the step's factor is E^2 of one field
                         op.e2f = (s.E2f && ep && ep2 == ep) ? s.E2f + (size_t)ch.step_field(t) * V : nullptr;
-                        op.rkey = rkey;
                         op.MsT = s.dz->MsT;
                         op.MaT = s.dz->MaT;
                         op.n = s.dz->n;
