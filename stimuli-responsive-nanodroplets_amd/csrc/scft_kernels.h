@@ -1525,6 +1525,313 @@ __global__ void __launch_bounds__(64 * kPlaneWaves) k_plane64(const PlaneOp *__r
     }
 }
 
+// ------------------------------- fused z + x step on 2D films, split by z columns
+//
+// One contour step of one chain on a 2D plane (NY = 1) of NX = L rows (x,
+// periodic, L = 192, 256 or 320) by NZ = 128 points (z, the dense operator
+// of k_zdense_mfma2: symmetric and antisymmetric halves of 64 x 64).  The
+// plane's step is split by z columns into kFilmParts workgroups: part
+// k0 = 16 part computes the z operator's symmetric output columns
+// k0..k0+15, i.e. the z points k0..k0+15 and n-16-k0..n-1-k0 (32 columns),
+// for every row, keeps them in LDS, then runs the x transforms of those 32
+// columns (16 complex lines) and stores them: the scratch between the two
+// axes never leaves the CU.  Every part reads the whole input plane (each
+// z output needs whole rows), so the parts of one plane go to one XCD
+// (block ids o % 8 + 8 part + 32 (o / 8): the dispatcher deals consecutive
+// workgroups to the eight XCDs in turn) and share the reads through its L2.
+// HBM then moves the input slice, the factor and the output slice once: 16
+// bytes per voxel-step in FP32-split, against 24 with the scratch slice
+// written by the z launch and read back by the x launch.
+//
+// The arithmetic is that of the two launches (k_zdense_mfma2 followed by
+// k_fftreg<16> for L = 256 or k_fftreg2<L/16, 16> otherwise): the same MFMA
+// operands in the same order, the scratch rounded to T, the same transform
+// steps and the same stores with the same rounding keys, so the results are
+// bit for bit those of the split step.
+
+constexpr int kFilmParts = 4;    // workgroups per plane and step
+constexpr int kFilmWaves = 4;
+constexpr int kFilmN = 128;      // NZ
+constexpr int kFilmCols = 32;    // z columns of one part (LDS row length, in T)
+
+struct FilmOp {
+    const void *src;       // slice t (T), x-major, z fastest
+    void *dst;             // slice t + 1 (T)
+    const double *epre;    // E of this step
+    const double *epre2;   // factor the stored slice owes (null, epre, or another E)
+    const float *e2f;      // E^2 in FP32 (inside a block, FP32-split with the FP32 factor on), or null
+    const double *Kx;      // x spectrum of this slot (length L)
+    unsigned rkey;         // unbiased-rounding key of the stored slice (store_round)
+};
+
+struct FilmTab {           // shared by every op of a launch
+    const double *MsT, *MaT;   // z operator halves, [64][ldm], MsT[j][k] = Ms[k][j]
+    const double *tw;          // w_L^{-e}, complex interleaved
+    int ldm;
+};
+
+// Slot of complex line i (local z columns 2i, 2i+1) in LDS row e.  The 16
+// slots of a row are permuted per row so that the rows one LDS access phase
+// touches (4 rows of 16-byte cells, or 8 rows of 8-byte cells) start in
+// different banks.
+template <typename T>
+__device__ __forceinline__ int film_slot(int e, int i)
+{
+    return i ^ (4 * ((sizeof(T) == 8 ? e : e >> 1) & 3));
+}
+
+// LDS offset (in T) of local column c of row e.
+template <typename T>
+__device__ __forceinline__ int film_at(int e, int c)
+{
+    return e * kFilmCols + 2 * film_slot<T>(e, c >> 1) + (c & 1);
+}
+
+// The z operator on rows rbase..rbase+15 for this part's 16 symmetric
+// output columns (mfma2_tile's loads, staging and fragments; bs/ba hold the
+// part's column of Ms^T / Ma^T for each K step); results rounded to T into
+// the part's LDS rows.  The slice is loaded as ordinary (cached) data: the
+// other parts of the plane read the same rows.
+template <typename T>
+__device__ __forceinline__ void film_ztile(const FilmOp &op, int rbase, const double (&bs)[16], const double (&ba)[16],
+                                           double *Xw, T *Tm, int lane)
+{
+    constexpr int n = kFilmN, NT = 16, XS = Mfma2<64>::XS;
+    const int li = lane & 15, lj = lane >> 4;
+    const int c = 2 * lane;   // this lane's point pair in a row
+    const T *src = static_cast<const T *>(op.src);
+    const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? (op.e2f ? 4 : 2) : 3);
+    Pair<T> xa[16];
+    Pair<double> ea[16];
+    const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(c);
+    if (mode == 4) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const unsigned a = o0 + unsigned(r) * unsigned(n);
+            xa[r] = ldg_pair(src, a);
+            const Pair<float> f = ldg_pair(op.e2f, a);
+            ea[r] = Pair<double>{double(f.x), double(f.y)};
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const unsigned a = o0 + unsigned(r) * unsigned(n);
+            xa[r] = ldg_pair(src, a);
+            ea[r] = ldg_pair(op.epre, a);
+        }
+    }
+    double sf[NT], af[NT];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        wave_sync();   // the previous reads of the staging rows are done
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int R = 8 * h + r;
+            double e0 = ea[R].x, e1 = ea[R].y;
+            if (mode == 2) {
+                e0 *= e0;
+                e1 *= e1;
+            } else if (mode == 3) {   // one step per block: the second factor
+                const Pair<double> f = ldg_pair(op.epre2, o0 + unsigned(R) * unsigned(n));
+                e0 *= f.x;
+                e1 *= f.y;
+            }
+            double2 v;
+            v.x = double(xa[R].x) * e0;
+            v.y = double(xa[R].y) * e1;
+            *reinterpret_cast<double2 *>(&Xw[r * XS + c]) = v;
+        }
+        wave_sync();
+        if ((li >> 3) == h) {
+            const double *row = Xw + (li & 7) * XS;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                const int j = lj * NT + k;
+                const double x1 = row[j], x2 = row[n - 1 - j];
+                sf[k] = x1 + x2;
+                af[k] = x1 - x2;
+            }
+        }
+    }
+    d4_t U = d4_t{0.0, 0.0, 0.0, 0.0}, W = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        U = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[k], bs[k], U, 0, 0, 0);
+        W = __builtin_amdgcn_mfma_f64_16x16x4f64(af[k], ba[k], W, 0, 0, 0);
+    }
+    // register r of lane l: row l/16 + 4r; output column li of the part
+    // (z = k0 + li) and its mirror (z = n-1-k0-li, local column 31 - li)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = rbase + lj + 4 * r;
+        Tm[film_at<T>(e, li)] = T(U[r] + W[r]);
+        Tm[film_at<T>(e, 31 - li)] = T(U[r] - W[r]);
+    }
+}
+
+// Transpose cells of one complex line: element (p, k) of the P x B tile
+// (lane p, lane-DFT output k) lives in the line's own LDS row
+// 16 k + (p ^ (k & 7)); the xor spreads the rows of one access phase over
+// the banks both when a column is written (p varies) and when a row is
+// read (k varies).
+__device__ __forceinline__ int film_cell(int p, int k)
+{
+    return 16 * k + (p ^ (k & 7));
+}
+
+// One exchange through a line's LDS cells: every lane of the line writes
+// a[k] to cell wc(k) for k < na, then reads b[j] from cell rc(j) for
+// j < nb (the caller's maps are a transpose, so no two lanes write one
+// cell).  FP64 rows hold a complex value per cell; FP32 rows hold one
+// double, so the real and the imaginary parts go through in two rounds.
+template <typename T, int NA, int NB, typename WC, typename RC>
+__device__ __forceinline__ void film_xchg(T *Tm, int i, const cpx<double> (&a)[NA], int na, WC wc, cpx<double> (&b)[NB],
+                                          int nb, RC rc)
+{
+    auto ptr = [&](int e) { return reinterpret_cast<double *>(Tm + e * kFilmCols + 2 * film_slot<T>(e, i)); };
+    wave_sync();   // earlier reads of these cells are done
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+            if (k < na) *reinterpret_cast<double2 *>(ptr(wc(k))) = double2{a[k].x, a[k].y};
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < nb) {
+                const double2 t = *reinterpret_cast<const double2 *>(ptr(rc(j)));
+                b[j] = cpx<double>{t.x, t.y};
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+            if (k < na) ptr(wc(k))[0] = a[k].x;
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < nb) b[j].x = ptr(rc(j))[0];
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+            if (k < na) ptr(wc(k))[0] = a[k].y;
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < nb) b[j].y = ptr(rc(j))[0];
+    }
+}
+
+// The x operator on complex line i of the part (local columns 2i, 2i+1:
+// z points z0, z0 + 1) from the part's LDS rows to the output slice.  Lane
+// p of the line's 16 holds x[p + 16 m], m < B = L/16: the steps of
+// freg_apply<16> (L = 256) or freg2_apply<B, 16>, with the tile transposes
+// through the line's own cells.
+template <int L, typename T>
+__device__ __forceinline__ void film_xline(const FilmOp &op, T *Tm, const cpx<double> *TW, int i, int p, int z0)
+{
+    constexpr int B = L / 16, R = (B + 15) / 16, n = kFilmN;
+    typedef T t2 __attribute__((ext_vector_type(2)));
+    cpx<double> v[B];
+#pragma unroll
+    for (int m = 0; m < B; ++m) {
+        const int e = p + 16 * m;
+        const t2 s = *reinterpret_cast<const t2 *>(Tm + e * kFilmCols + 2 * film_slot<T>(e, i));
+        v[m] = cpx<double>{double(s.x), double(s.y)};
+    }
+    // forward: over m, twiddle w^{-p k2} (p k2 < L)
+    if constexpr (L == 256) dft16<-1>(v);
+    else dft_lane<B, -1>(v);
+#pragma unroll
+    for (int k = 1; k < B; ++k) {
+        const int e = p * k;
+        v[k] = cmul(v[k], TW[e + (e >> 4)]);
+    }
+    // lane q takes the columns k2 = q + 16 r < B of the tile
+    const int q = p;
+    const int ncol = (q + 16 < B) ? 2 : (q < B ? 1 : 0);
+    cpx<double> col[16 * R];
+    film_xchg<T>(
+        Tm, i, v, B, [&](int k) { return film_cell(p, k); }, col, 16 * ncol,
+        [&](int j) { return film_cell(j & 15, q + 16 * (j >> 4)); });
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int c = q + 16 * r;
+        if (c < B) {
+            dft16<-1>(col, 16 * r);   // over p: spectrum index c + B k1
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) {
+                const double kk = ldg(op.Kx, unsigned(c + B * k1)) * (1.0 / L);   // 1/L folded in (exact)
+                col[16 * r + k1].x *= kk;
+                col[16 * r + k1].y *= kk;
+            }
+            dft16<+1>(col, 16 * r);
+        }
+    }
+    if constexpr (L == 256) {   // freg_apply twiddles back before the transpose
+#pragma unroll
+        for (int k = 1; k < 16; ++k) {
+            const int e = k * q;
+            cpx<double> w = TW[e + (e >> 4)];
+            w.y = -w.y;
+            col[k] = cmul(col[k], w);
+        }
+    }
+    film_xchg<T>(
+        Tm, i, col, 16 * ncol, [&](int j) { return film_cell(j & 15, q + 16 * (j >> 4)); }, v, B,
+        [&](int k) { return film_cell(p, k); });
+    if constexpr (L != 256) {   // freg2_apply twiddles back after it
+#pragma unroll
+        for (int k = 1; k < B; ++k) {
+            const int e = p * k;
+            cpx<double> w = TW[e + (e >> 4)];
+            w.y = -w.y;
+            v[k] = cmul(v[k], w);
+        }
+    }
+    if constexpr (L == 256) dft16<+1>(v);   // v[m] = x[p + 16 m] after the operator
+    else dft_lane<B, +1>(v);
+    T *dst = static_cast<T *>(op.dst);
+#pragma unroll
+    for (int m = 0; m < B; ++m) {
+        const unsigned a0 = unsigned(p + 16 * m) * unsigned(n) + unsigned(z0);
+        stg_pair(dst, a0, store_round<T>(v[m].x, op.rkey, a0), store_round<T>(v[m].y, op.rkey, a0 + 1));
+    }
+}
+
+// grid: 32 ceil(nops / 8) workgroups; block b runs part (b / 8) % 4 of op
+// 8 (b / 32) + b % 8
+template <int L, typename T>
+__global__ void __launch_bounds__(64 * kFilmWaves) k_film(const FilmOp *__restrict__ ops, int nops, FilmTab tab)
+{
+    constexpr int NTILE = L / 16, XS = Mfma2<64>::XS;
+    __shared__ __attribute__((aligned(16))) T Tm[L * kFilmCols];
+    __shared__ __attribute__((aligned(16))) double X[kFilmWaves * 8 * XS];
+    __shared__ cpx<double> TW[L + L / 16];
+    const int b = blockIdx.x;
+    const int o = (b >> 5) * 8 + (b & 7), part = (b >> 3) & 3;
+    if (o >= nops) return;   // the whole workgroup: no barrier follows
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const FilmOp &op = ops[o];
+    for (int e = tid; e < L; e += 64 * kFilmWaves)
+        TW[e + (e >> 4)] = cpx<double>{ldg(tab.tw, unsigned(2 * e)), ldg(tab.tw, unsigned(2 * e + 1))};
+    // the part's column of Ms^T / Ma^T for every K step (j = lj 16 + k)
+    const int li = lane & 15, lj = lane >> 4;
+    double bs[16], ba[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const unsigned a = unsigned((lj * 16 + k) * tab.ldm + 16 * part + li);
+        bs[k] = ldg(tab.MsT, a);
+        ba[k] = ldg(tab.MaT, a);
+    }
+    double *Xw = X + wave * 8 * XS;
+    for (int t = wave; t < NTILE; t += kFilmWaves) film_ztile<T>(op, 16 * t, bs, ba, Xw, Tm, lane);
+    __syncthreads();   // every row of the part's columns is in LDS
+    const int g = lane & 3, p = lane >> 2;
+    const int i = 4 * wave + g;   // complex line: local columns 2i, 2i + 1
+    const int z0 = i < 8 ? 16 * part + 2 * i : kFilmN - kFilmCols - 16 * part + 2 * i;
+    film_xline<L, T>(op, Tm, TW, i, p, z0);
+}
+
 // ------------------------------------------------------------ noise stream
 //
 // The reference's deviates (funGaussDev, SCFT.c:1769-1782): POSIX drand48,

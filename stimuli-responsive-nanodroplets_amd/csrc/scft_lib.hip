@@ -162,9 +162,9 @@ enum ZKind { Z_FFT = 0, Z_DENSE = 1 };
 constexpr int kE2fMinV = 16384;
 
 // Process-wide autotuner results: (n, dirichlet, alpha bits, storage,
-// slots timed) -> ZKind.
+// slots timed, NX, candidate set) -> candidate (autotune_z).
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, long long, int, int>, int> g_tune;
+std::map<std::tuple<int, int, long long, int, int, int, int>, int> g_tune;
 
 // The autotuner times as many slots as put this many voxels in one launch
 // (at most the batch): one small slot alone times launch latency only.
@@ -217,9 +217,9 @@ struct Slot {
 
 enum LaunchKind {
     LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFTREG = 5, LK_FFTREG2 = 6,
-    LK_PLANE = 7
+    LK_PLANE = 7, LK_FILM = 8
 };
-constexpr int kLaunchKinds = 8;
+constexpr int kLaunchKinds = 9;
 
 struct Launch {
     int kind;        // LaunchKind
@@ -228,6 +228,7 @@ struct Launch {
     int L = 0;       // transform length (FFT kinds)
     int hsp = 0;     // padded dense half width (MFMA kind)
     size_t lds;
+    FilmTab ftab{};  // tables of a fused film step (LK_FILM)
 };
 
 bool fftwave_supported(int L)
@@ -274,6 +275,10 @@ struct scft_ctx {
     PlaneOp *planeops = nullptr;
     size_t planeops_cap = 0;
     bool plane_on = true;            // fused step on periodic 64 x 64 planes (SCFT_PLANE=0: off)
+    FilmOp *filmops = nullptr;
+    size_t filmops_cap = 0;
+    int film_mode = -1;              // fused z + x step on 2D films: 0 off, 1 on, -1 autotuned (SCFT_FILM)
+    bool film_cls[3] = {};           // film step chosen for active sets of <= 16, <= 64, more slots
     int nstep0 = 0;                  // launches of contour step 0
     bool inflight = false;           // a batched iteration launched by scft_batch_launch, not yet completed
     PrepArgs *d_prep = nullptr;
@@ -705,6 +710,23 @@ unsigned round_key(const scft_ctx *c, int ci, int t, int ai, bool last)
     return k ? k : 1u;
 }
 
+// x lengths of the fused film step (the register-FFT forms of 16 lanes)
+bool film_length(int L) { return L == 192 || L == 256 || L == 320; }
+
+// The film step against the split step is tuned for three sizes of the
+// active set: it wins where few chains share a launch (its fused launch
+// halves the launches and keeps the scratch on chip) and can lose on a full
+// batch (each part re-stages whole rows for a quarter of the products).
+constexpr int kFilmTuneSlots[3] = {8, 32, 128};
+int film_class(int nact) { return nact <= 16 ? 0 : (nact <= 64 ? 1 : 2); }
+
+// The context's kernels allow the fused film step: 2D (NY = 1), NZ = 128
+// on the even MFMA dense operator, the scratch in the slice type.
+bool film_fits(const scft_ctx *c)
+{
+    return c->NY == 1 && c->NZ == kFilmN && c->dense_mfma && !c->dense_mfma1 && !c->tmp_wide && c->zkind == Z_DENSE;
+}
+
 int build_schedule(scft_ctx *c)
 {
     drop_graph(c);
@@ -716,6 +738,7 @@ int build_schedule(scft_ctx *c)
     std::vector<LineOp> lops;
     std::vector<DenseOp> dops;
     std::vector<PlaneOp> pops;
+    std::vector<FilmOp> fops;
     int maxsteps = 0;
     for (int si : c->act)
         for (auto &ch : c->slots[si].chains) maxsteps = std::max(maxsteps, ch.nsteps);
@@ -723,7 +746,57 @@ int build_schedule(scft_ctx *c)
     // every active slot a periodic 64 x 64 plane: one fused launch per step
     bool plane = c->plane_on && c->fft_reg && c->zkind == Z_FFT && c->NY == 1 && c->NZ == kPlaneN;
     for (int si : c->act) plane &= c->slots[si].NX == kPlaneN && !c->slots[si].p.mask;
+    // every active slot a 2D film with the dense z operator of NZ = 128 and
+    // a register-FFT x length: one fused launch per step and x length
+    bool film = film_fits(c) && !c->act.empty() && c->film_cls[film_class(int(c->act.size()))];
+    std::vector<int> film_lens;
+    for (int si : c->act) {
+        const Slot &s = c->slots[si];
+        film &= film_length(s.NX) && s.dz && s.dz->hs == kFilmN / 2 && s.dz->ha == kFilmN / 2 &&
+                s.dz->MsT == c->slots[c->act[0]].dz->MsT;
+        if (std::find(film_lens.begin(), film_lens.end(), s.NX) == film_lens.end()) film_lens.push_back(s.NX);
+    }
     for (int t = 0; t < maxsteps; ++t) {
+        if (film) {
+            for (int L : film_lens) {
+                Launch ln{};
+                ln.kind = LK_FILM;
+                ln.pos = 3;
+                ln.L = L;
+                ln.first = int(fops.size());
+                const Slot *s0 = nullptr;
+                for (int si : c->act) {
+                    Slot &s = c->slots[si];
+                    if (s.NX != L) continue;
+                    if (!s0) s0 = &s;
+                    const size_t V = s.V;
+                    for (size_t ci = 0; ci < s.chains.size(); ++ci) {
+                        const Chain &ch = s.chains[ci];
+                        if (ch.nsteps <= t) continue;
+                        FilmOp op{};
+                        op.src = static_cast<const char *>(ch.base) + (size_t)t * V * ts;
+                        op.dst = static_cast<char *>(ch.base) + (size_t)(t + 1) * V * ts;
+                        op.epre = s.E + (size_t)ch.step_field(t) * V;
+                        op.epre2 = slice_factor(s, ch, t);
+                        op.e2f = (s.E2f && op.epre2 == op.epre) ? s.E2f + (size_t)ch.step_field(t) * V : nullptr;
+                        op.Kx = s.fx->K;
+                        // the key of the split step's x launch (axis 1 of z, x)
+                        op.rkey = round_key(c, int(ci), t, 1, true);
+                        fops.push_back(op);
+                    }
+                }
+                ln.count = int(fops.size()) - ln.first;
+                if (ln.count == 0) continue;
+                ln.ftab.MsT = s0->dz->MsT;
+                ln.ftab.MaT = s0->dz->MaT;
+                ln.ftab.ldm = s0->dz->ldm;
+                ln.ftab.tw = reinterpret_cast<const double *>(s0->fx->tw);
+                ln.gx = 32 * ((ln.count + 7) / 8);   // kFilmParts parts of every op, XCD-grouped
+                c->launches.push_back(ln);
+            }
+            if (t == 0) c->nstep0 = int(c->launches.size());
+            continue;
+        }
         if (plane) {
             Launch ln{};
             ln.kind = LK_PLANE;
@@ -916,6 +989,7 @@ int build_schedule(scft_ctx *c)
     RETCHK(upload(c, &c->lineops, &c->lineops_cap, lops));
     RETCHK(upload(c, &c->denseops, &c->denseops_cap, dops));
     RETCHK(upload(c, &c->planeops, &c->planeops_cap, pops));
+    RETCHK(upload(c, &c->filmops, &c->filmops_cap, fops));
     // per-slot argument arrays in active order
     std::vector<PrepArgs> prep;
     std::vector<DenArgs> den;
@@ -1025,6 +1099,15 @@ void launch_one(scft_ctx *c, const Launch &ln)
         hipLaunchKernelGGL((k_plane64<Tin, Tout>), dim3(ln.gx), dim3(64 * kPlaneWaves), 0, st,
                            c->planeops + ln.first, ln.count);
         break;
+    case LK_FILM: {
+        // slices and the on-chip scratch are T (the film step is never scheduled with a wide scratch)
+        const FilmOp *ops = c->filmops + ln.first;
+        const dim3 g1(ln.gx), b(64 * kFilmWaves);
+        if (ln.L == 256) hipLaunchKernelGGL((k_film<256, Tin>), g1, b, 0, st, ops, ln.count, ln.ftab);
+        else if (ln.L == 192) hipLaunchKernelGGL((k_film<192, Tin>), g1, b, 0, st, ops, ln.count, ln.ftab);
+        else hipLaunchKernelGGL((k_film<320, Tin>), g1, b, 0, st, ops, ln.count, ln.ftab);
+        break;
+    }
     case LK_DENSE_MFMA: {
         const DenseOp *ops = c->denseops + ln.first;
         const dim3 g1(ln.gx), b(64 * kMfmaWaves);
@@ -1246,63 +1329,104 @@ int autotune_z(scft_ctx *c)
     if (scft_derive(&p, &d0) != SCFT_OK) return SCFT_EINVAL;
     const long long v0 = std::max(1LL, (long long)d0.nx * d0.ny * d0.nz);
     const int nt = int(std::min<long long>(c->nslots, std::max(1LL, (kTuneVoxels + v0 - 1) / v0)));
-    auto key = std::make_tuple(c->NZ, p.mask ? 1 : 0, abits, int(c->tsize), nt);
     const int hs = c->NZ - c->NZ / 2;
     const bool dense_ok = hs <= kDenseHMax;
-    // SCFT_ZKIND=fft|dense forces the z operator (tests use it to check both)
+    // candidates: 0 the z transform, 1 the dense z operator (z and x
+    // launches apart), 2 the dense z operator fused with x (film step)
+    c->zkind = Z_DENSE;
+    const bool film_ok = dense_ok && film_fits(c) && film_length(d0.nx) && c->film_mode != 0;
+    c->zkind = Z_FFT;
+    // SCFT_ZKIND=fft|dense forces the z operator (tests use it to check both);
+    // SCFT_FILM=0|1 keeps the dense operator's two launches or fuses them
     const char *env = std::getenv("SCFT_ZKIND");
-    if (env && std::strcmp(env, "fft") == 0) {
-        c->zkind = Z_FFT;
+    const bool force_fft = env && std::strcmp(env, "fft") == 0;
+    const bool force_dense = env && std::strcmp(env, "dense") == 0 && dense_ok;
+    std::vector<int> cands;
+    if (!force_dense) cands.push_back(0);
+    if (dense_ok && !force_fft) {
+        if (!(film_ok && c->film_mode == 1)) cands.push_back(1);
+        if (film_ok) cands.push_back(2);
+    }
+    // result: bits 0-1 the candidate at nt slots, bit 2 + k the film step for size class k
+    auto apply = [&](int r) {
+        const int k = r & 3;
+        c->zkind = k == 0 ? Z_FFT : Z_DENSE;
+        for (int i = 0; i < 3; ++i) c->film_cls[i] = k != 0 && ((r >> (2 + i)) & 1);
+    };
+    if (cands.size() == 1) {
+        apply(cands[0] == 2 ? (2 | (7 << 2)) : cands[0]);
         return SCFT_OK;
     }
-    if (env && std::strcmp(env, "dense") == 0 && dense_ok) {
-        c->zkind = Z_DENSE;
-        return SCFT_OK;
-    }
+    auto key = std::make_tuple(c->NZ, p.mask ? 1 : 0, abits, int(c->tsize), nt, d0.nx, int(cands.size()) * 4 + cands[0]);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
-        if (it != g_tune.end()) { c->zkind = it->second; return SCFT_OK; }
+        if (it != g_tune.end()) { apply(it->second); return SCFT_OK; }
     }
-    int choice = Z_FFT;
-    if (dense_ok) {
-        double best = 1e30;
-        for (int kind : {Z_FFT, Z_DENSE}) {
-            c->zkind = kind;
-            for (int t = 0; t < nt; ++t) {
+    // mean time of the launches of contour step 0 with `ns` slots active (the
+    // x launch is the same for the z kinds unless a fused step replaces both axes)
+    auto time_step0 = [&](int kind, int ns, double *ms_out) -> int {
+        c->zkind = kind == 0 ? Z_FFT : Z_DENSE;
+        for (int i = 0; i < 3; ++i) c->film_cls[i] = kind == 2;
+        for (int t = 0; t < c->nslots; ++t) {
+            if (t < ns) {
                 RETCHK(slot_init(c, c->slots[t], c->proto));
                 c->slots[t].active = true;
+            } else {
+                c->slots[t].active = false;
             }
-            RETCHK(build_schedule(c));
-            // time the launches of step 0 (the x launch is the same for both
-            // kinds unless the fused plane step replaces both axes)
-            hipEvent_t e0, e1;
-            HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventCreate(&e1));
-            std::vector<Launch> zl(c->launches.begin(), c->launches.begin() + c->nstep0);
-            std::swap(zl, c->launches);
-            const bool f32 = c->tsize == 4;
-            for (int w = 0; w < 3; ++w) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
-            HIPCHK(hipEventRecord(e0, c->stream));
-            for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
-            HIPCHK(hipEventRecord(e1, c->stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            std::swap(zl, c->launches);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            if (ms < best) { best = ms; choice = kind; }
         }
-        for (int t = 0; t < nt; ++t) {
-            c->slots[t].active = false;
-            c->slots[t].ready = false;
-        }
-        c->dirty = true;
+        RETCHK(build_schedule(c));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        std::vector<Launch> zl(c->launches.begin(), c->launches.begin() + c->nstep0);
+        std::swap(zl, c->launches);
+        const bool f32 = c->tsize == 4;
+        for (int w = 0; w < 3; ++w) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        std::swap(zl, c->launches);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_out = ms;
+        return SCFT_OK;
+    };
+    int choice = cands[0];
+    double best = 1e30;
+    for (int kind : cands) {
+        double ms;
+        RETCHK(time_step0(kind, nt, &ms));
+        if (ms < best) { best = ms; choice = kind; }
     }
-    c->zkind = choice;
+    int result = choice | (choice == 2 ? (4 << 2) : 0);
+    // the film step for the smaller size classes, timed against the split step
+    const bool has_film = std::find(cands.begin(), cands.end(), 2) != cands.end();
+    const bool has_split = std::find(cands.begin(), cands.end(), 1) != cands.end();
+    for (int k = 0; k < 2; ++k) {
+        if (choice == 0) break;
+        const int ns = std::min(kFilmTuneSlots[k], nt);
+        bool f = choice == 2;
+        if (ns < nt && has_film && has_split) {
+            double t1, t2;
+            RETCHK(time_step0(1, ns, &t1));
+            RETCHK(time_step0(2, ns, &t2));
+            f = t2 < t1;
+        }
+        if (f) result |= 1 << (2 + k);
+    }
+    for (int t = 0; t < c->nslots; ++t) {
+        c->slots[t].active = false;
+        c->slots[t].ready = false;
+    }
+    c->dirty = true;
+    apply(result);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = choice;
+    g_tune[key] = result;
     return SCFT_OK;
 }
 
@@ -1365,6 +1489,8 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     c->fft_reg_force = xf && std::strcmp(xf, "reg") == 0;
     const char *pl = std::getenv("SCFT_PLANE");
     c->plane_on = !(pl && pl[0] == '0');
+    const char *fm = std::getenv("SCFT_FILM");
+    c->film_mode = !fm ? -1 : (fm[0] == '0' ? 0 : 1);
     const char *dm = std::getenv("SCFT_DENSE");
     c->dense_mfma = !(dm && std::strcmp(dm, "valu") == 0);
     c->dense_mfma1 = dm && std::strcmp(dm, "mfma1") == 0;
@@ -1893,7 +2019,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
     const double ts = double(c->tsize);
     for (const Launch &ln : c->launches) {
         const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
-                          ln.kind == LK_PLANE ||
+                          ln.kind == LK_PLANE || ln.kind == LK_FILM ||
                           ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFTREG ||
                             ln.kind == LK_FFTREG2) && ln.pos == 0);
         if ((kind == 0) != is_z) continue;
@@ -1908,7 +2034,18 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
             if (e2f) return 4.0 + (ep ? 8.0 : 0.0);
             return (e1 ? (e2 && e2 != e1 ? 16.0 : 8.0) : 0.0) + (ep ? 8.0 : 0.0);
         };
-        if (ln.kind == LK_PLANE) {
+        if (ln.kind == LK_FILM) {
+            std::vector<FilmOp> ops(ln.count);
+            HIPCHK(hipMemcpy(ops.data(), c->filmops + ln.first, ln.count * sizeof(FilmOp), hipMemcpyDeviceToHost));
+            for (auto &op : ops) {
+                const double elems = double(ln.L) * kFilmN;
+                U += elems;
+                B += elems * (in_b + out_b + e_bytes(op.epre, op.epre2, nullptr, op.e2f));
+                // z: NZ^2/2 FMA per line of NZ; x: 2 x 5 L log2 L per packed line pair, NZ/2 pairs
+                Fl += 2.0 * ln.L * (2.0 * (kFilmN / 2.0) * (kFilmN / 2.0)) +
+                      (kFilmN / 2.0) * 2.0 * 5.0 * ln.L * std::log2(double(ln.L));
+            }
+        } else if (ln.kind == LK_PLANE) {
             std::vector<PlaneOp> ops(ln.count);
             HIPCHK(hipMemcpy(ops.data(), c->planeops + ln.first, ln.count * sizeof(PlaneOp), hipMemcpyDeviceToHost));
             for (auto &op : ops) {
@@ -1976,9 +2113,10 @@ int scft_get_schedule(const scft_ctx *c, char *buf, int len)
         if (ln.kind >= 0 && ln.kind < kLaunchKinds) n[ln.kind]++;
     std::snprintf(buf, len,
                   "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fftreg=%d dense_mfma=%d dense_mfma2=%d graph=%d "
-                  "fftreg2=%d plane=%d",
+                  "fftreg2=%d plane=%d film=%d",
                   c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFTREG],
-                  n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0, n[LK_FFTREG2], n[LK_PLANE]);
+                  n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0, n[LK_FFTREG2], n[LK_PLANE],
+                  n[LK_FILM]);
     return SCFT_OK;
 }
 

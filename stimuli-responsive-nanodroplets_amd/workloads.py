@@ -4,6 +4,9 @@ last-axis transform (HIP events, scft_time_launches).
 
     python workloads.py c1 [--slots B] [--iters N] [--precision fp32|fp64]
         2D bulk AB diblock melt, 64x1x64, NS 101, MASK 0; B resident (1)
+    python workloads.py c2 [--slots B] [--iters N] [--precision fp32|fp64]
+        2D thin-film droplet (the bench workload), 256x1x128, NS 101,
+        MASK 1; B resident (192)
     python workloads.py c3 [--slots B] [--iters N] [--precision fp32|fp64]
         3D thin-film droplet, 128x128x64, NS 201 (C3 chemistry); B resident (1)
     python workloads.py c5 [--slots B] [--iters N] [--precision ...]
@@ -47,6 +50,10 @@ import configs  # noqa: E402
 
 def c1_params(slots):
     return [configs.params_c1() for _ in range(slots)]
+
+
+def c2_params(slots):
+    return [configs.params_c2() for _ in range(slots)]
 
 
 def c3_params(slots):
@@ -155,7 +162,7 @@ def run(lib, name, protos, precision, iters):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=["c1", "c3", "c4", "c5", "c5stream"])
+    ap.add_argument("workload", choices=["c1", "c2", "c3", "c4", "c5", "c5stream"])
     ap.add_argument("--rates", default="2,4,6", help="c5stream: arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=40.0, help="c5stream: seconds of arrivals per rate")
     ap.add_argument("--scale", type=float, default=1e-3, help="c5stream: iteration budget scale")
@@ -169,9 +176,11 @@ def main(argv=None):
         for r in a.rates.split(","):
             c5_stream(lib, float(r), a.duration, a.scale, prec)
         return
-    slots = a.slots or {"c1": 1, "c3": 1, "c4": 192, "c5": 8}[a.workload]
+    slots = a.slots or {"c1": 1, "c2": 192, "c3": 1, "c4": 192, "c5": 8}[a.workload]
     if a.workload == "c1":
         run(lib, "C1 64x1x64", c1_params(slots), prec, a.iters)
+    elif a.workload == "c2":
+        run(lib, "C2 256x1x128", c2_params(slots), prec, a.iters)
     elif a.workload == "c3":
         run(lib, "C3 128x128x64", c3_params(slots), prec, a.iters)
     elif a.workload == "c5":

@@ -101,27 +101,40 @@ def test_3d_config_matches_golden(abi, configs, lib, which, prec):
     ctx.close()
 
 
+@pytest.mark.parametrize("form", ["auto", "split", "film"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_c4_sweep_batch_matches_golden(abi, configs, lib, prec):
+def test_c4_sweep_batch_matches_golden(abi, configs, lib, prec, form):
     """The whole 192-slot C4 batch (chi_AB N 15..36 x chi_AC N 20..55, NX
-    192/256/320 drawn per slot, one batch, one launch per contour step over
-    every slot): the NX 192 and NX 320 fixture slots match the oracle and
-    slots with the same parameters and seed agree bit for bit."""
+    192/256/320 drawn per slot, one batch, one z launch per contour step over
+    every slot, or one fused film launch per step and x length): the NX 192
+    and NX 320 fixture slots match the oracle and slots with the same
+    parameters and seed agree bit for bit."""
+    import os
     import workloads
     g = _golden("c4")
     protos = workloads.c4_params(192)
     precision = abi.SCFT_FP64 if prec == "fp64" else abi.SCFT_FP32_SPLIT
-    b = abi.Batch(lib, protos[0], 192, max_nx=320, precision=precision)
+    if form != "auto":
+        os.environ["SCFT_ZKIND"] = "dense"
+        os.environ["SCFT_FILM"] = "1" if form == "film" else "0"
+    try:
+        b = abi.Batch(lib, protos[0], 192, max_nx=320, precision=precision)
+    finally:
+        os.environ.pop("SCFT_ZKIND", None)
+        os.environ.pop("SCFT_FILM", None)
     for s, p in enumerate(protos):
         b.init_slot(s, p)
         w, pf = configs.droplet_fields(p, radius=6.0, seed=s % 64)
         b.set_fields(s, w, pf)
         b.set_mask(s, configs.brush_mask(p))
         b.activate(s)
-    print(b.schedule(), flush=True)
     iters = [row["iter"] for row in next(iter(g.values()))["iters"]]
     # Batch.iterate returns its reused out array: keep a copy per iteration
     outs = [[abi.ScftIterOut.from_buffer_copy(o) for o in b.iterate([it] * 192)] for it in iters]
+    print(b.schedule(), flush=True)
+    if form != "auto":   # the schedule exists once an iteration ran
+        nfilm = int(b.schedule().split("film=")[1].split()[0])
+        assert (nfilm > 0) == (form == "film"), b.schedule()
     for key, fx in g.items():
         s = fx["slot"]
         assert abi.grid_of(protos[s])[0] == int(key.split("nx")[1])

@@ -205,14 +205,18 @@ def _golden():
         return json.load(f)
 
 
-@pytest.mark.parametrize("zkind", ["auto", "fft", "dense"])
+@pytest.mark.parametrize("zkind", ["auto", "fft", "dense", "film"])
 def test_c2_full_size_matches_golden(abi, configs, lib, zkind):
     """C2 (256x1x128, MASK 1, NS 101) against the oracle's FP64 fixture
-    (tests/golden/make_golden.py): three iterations, 1e-10."""
+    (tests/golden/make_golden.py): three iterations, 1e-10.  "dense" runs
+    the dense z operator and the x transform as two launches per step,
+    "film" as one fused launch (k_film)."""
     import os
     g = _golden()["c2_droplet_seed0"]
     if zkind != "auto":
-        os.environ["SCFT_ZKIND"] = zkind
+        os.environ["SCFT_ZKIND"] = "dense" if zkind == "film" else zkind
+    if zkind in ("dense", "film"):
+        os.environ["SCFT_FILM"] = "1" if zkind == "film" else "0"
     try:
         p = configs.params_c2()
         ctx = abi.Context(lib, p, abi.SCFT_FP64)
@@ -227,6 +231,9 @@ def test_c2_full_size_matches_golden(abi, configs, lib, zkind):
             assert _scalar_rel(o.dw_rms, row["dw_rms"]) <= TOL
             for sp in range(17):
                 assert _scalar_rel(o.chempot[sp], row["chempot"][sp]) <= TOL
+        if zkind in ("dense", "film"):   # the schedule exists once an iteration ran
+            nfilm = int(ctx.schedule().split("film=")[1].split()[0])
+            assert (nfilm > 0) == (zkind == "film"), ctx.schedule()
         wf, pfin = ctx.get_fields()
         phi, _ = ctx.get_densities()
         idx = g["sample_idx"]
@@ -239,6 +246,7 @@ def test_c2_full_size_matches_golden(abi, configs, lib, zkind):
         ctx.close()
     finally:
         os.environ.pop("SCFT_ZKIND", None)
+        os.environ.pop("SCFT_FILM", None)
 
 
 def test_c2_fp32_split_batch_matches_golden(abi, configs, lib):
