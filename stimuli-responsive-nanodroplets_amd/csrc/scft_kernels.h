@@ -1726,11 +1726,11 @@ __device__ __forceinline__ void film_xchg(T *Tm, int i, const cpx<double> (&a)[N
 // freg_apply<16> (L = 256) or freg2_apply<B, 16>, with the tile transposes
 // through the line's own cells.
 template <int L, typename T>
-__device__ __forceinline__ void film_xline(const FilmOp &op, T *Tm, const cpx<double> *TW, int i, int p, int z0)
+__device__ __forceinline__ void film_xops(const double *Kx, T *Tm, const cpx<double> *TW, int i, int p,
+                                          cpx<double> (&v)[L / 16])
 {
-    constexpr int B = L / 16, R = (B + 15) / 16, n = kFilmN;
+    constexpr int B = L / 16, R = (B + 15) / 16;
     typedef T t2 __attribute__((ext_vector_type(2)));
-    cpx<double> v[B];
 #pragma unroll
     for (int m = 0; m < B; ++m) {
         const int e = p + 16 * m;
@@ -1759,7 +1759,7 @@ __device__ __forceinline__ void film_xline(const FilmOp &op, T *Tm, const cpx<do
             dft16<-1>(col, 16 * r);   // over p: spectrum index c + B k1
 #pragma unroll
             for (int k1 = 0; k1 < 16; ++k1) {
-                const double kk = ldg(op.Kx, unsigned(c + B * k1)) * (1.0 / L);   // 1/L folded in (exact)
+                const double kk = ldg(Kx, unsigned(c + B * k1)) * (1.0 / L);   // 1/L folded in (exact)
                 col[16 * r + k1].x *= kk;
                 col[16 * r + k1].y *= kk;
             }
@@ -1789,11 +1789,85 @@ __device__ __forceinline__ void film_xline(const FilmOp &op, T *Tm, const cpx<do
     }
     if constexpr (L == 256) dft16<+1>(v);   // v[m] = x[p + 16 m] after the operator
     else dft_lane<B, +1>(v);
+}
+
+// The film step's x operator on complex line i, stored to the output slice.
+template <int L, typename T>
+__device__ __forceinline__ void film_xline(const FilmOp &op, T *Tm, const cpx<double> *TW, int i, int p, int z0)
+{
+    constexpr int B = L / 16, n = kFilmN;
+    cpx<double> v[B];
+    film_xops<L, T>(op.Kx, Tm, TW, i, p, v);
     T *dst = static_cast<T *>(op.dst);
 #pragma unroll
     for (int m = 0; m < B; ++m) {
         const unsigned a0 = unsigned(p + 16 * m) * unsigned(n) + unsigned(z0);
         stg_pair(dst, a0, store_round<T>(v[m].x, op.rkey, a0), store_round<T>(v[m].y, op.rkey, a0 + 1));
+    }
+}
+
+// ------------------------------------- periodic lines through LDS, 32 at a time
+//
+// The last (periodic) axis of a step for 32 lines whose starts are 32
+// consecutive elements (x lines, or y lines of a 3D grid, in groups within
+// a row), of length L = 16 B (128, 192, 256, 320).  The workgroup loads
+// the 32 columns of its L rows with coalesced 16-byte row-segment loads
+// into LDS (the layout of the film step's part rows), runs the film step's
+// x transform on its 16 complex lines (4 waves x 4 lines of 16 lanes),
+// writes the rounded results back into the same LDS rows, and stores them
+// with the same coalesced row segments.  Against the register FFT's
+// strided loads (a 32-byte piece of 16 rows per instruction), every memory
+// instruction here moves whole 128-byte lines.
+
+template <int L, typename T>
+__global__ void __launch_bounds__(64 * kFilmWaves) k_fftx(const LineOp *__restrict__ ops, int gpo)
+{
+    constexpr int E4 = 16 / int(sizeof(T));        // elements per 16-byte segment
+    constexpr int LPR = kFilmCols / E4;            // lanes per row
+    constexpr int RPR = 64 * kFilmWaves / LPR;     // rows per round
+    static_assert(L % RPR == 0, "rows per round divide the length");
+    typedef T tv __attribute__((ext_vector_type(E4)));
+    __shared__ __attribute__((aligned(16))) T Tm[L * kFilmCols];
+    __shared__ cpx<double> TW[L + L / 16];
+    const LineOp &op = ops[blockIdx.x / gpo];
+    const int l0 = kFilmCols * (blockIdx.x % gpo);
+    if (l0 >= op.nlines) return;   // the whole workgroup: no barrier follows
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const double *twd = reinterpret_cast<const double *>(op.tw);
+    for (int e = tid; e < L; e += 64 * kFilmWaves)
+        TW[e + (e >> 4)] = cpx<double>{ldg(twd, unsigned(2 * e)), ldg(twd, unsigned(2 * e + 1))};
+    const unsigned base = unsigned(line_base(op, l0)), pitch = unsigned(op.stride);
+    const int rr = tid / LPR, c = E4 * (tid % LPR);   // this thread's row in a round and first column
+    const T *src = static_cast<const T *>(op.src);
+#pragma unroll
+    for (int k = 0; k < L / RPR; ++k) {
+        const int e = rr + RPR * k;
+        const tv x = *(const __attribute__((address_space(1))) tv *)((const char *)src +
+                                                                     (base + unsigned(e) * pitch + unsigned(c)) * sizeof(T));
+        *reinterpret_cast<tv *>(Tm + e * kFilmCols + 2 * film_slot<T>(e, c >> 1)) = x;
+    }
+    __syncthreads();
+    const int g = lane & 3, p = lane >> 2;
+    const int i = 4 * wave + g;   // complex line: columns 2i, 2i + 1
+    cpx<double> v[L / 16];
+    film_xops<L, T>(op.K, Tm, TW, i, p, v);
+    wave_sync();   // the line's last reads of its cells are done
+#pragma unroll
+    for (int m = 0; m < L / 16; ++m) {
+        const int e = p + 16 * m;
+        const unsigned a0 = base + unsigned(e) * pitch + unsigned(2 * i);
+        T *cell = Tm + e * kFilmCols + 2 * film_slot<T>(e, i);
+        cell[0] = store_round<T>(v[m].x, op.rkey, a0);
+        cell[1] = store_round<T>(v[m].y, op.rkey, a0 + 1);
+    }
+    __syncthreads();
+    T *dst = static_cast<T *>(op.dst);
+#pragma unroll
+    for (int k = 0; k < L / RPR; ++k) {
+        const int e = rr + RPR * k;
+        const tv x = *reinterpret_cast<const tv *>(Tm + e * kFilmCols + 2 * film_slot<T>(e, c >> 1));
+        *(__attribute__((address_space(1))) tv *)((char *)dst + (base + unsigned(e) * pitch + unsigned(c)) * sizeof(T)) = x;
     }
 }
 
