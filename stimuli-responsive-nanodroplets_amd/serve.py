@@ -23,6 +23,7 @@ the reference's noise.
 import collections
 import ctypes as C
 import math
+import os
 import time
 
 import numpy as np
@@ -486,9 +487,23 @@ class Server:
                 if other.inflight:
                     other.step_complete()
             nx = grid_of(req.params)[0]
-            pool = Pool(self.lib, req.params, precision=self.precision, max_slots=self.slots.get(key, self.default_slots),
-                        max_nx=max(self.max_nx, nx), hbm_fraction=self.hbm_fraction, tol=self.tol,
-                        max_iters=self.max_iters)
+            # Pools of different structures run concurrently here.  The fused
+            # film step wins on an idle GPU where few chains share a launch,
+            # but next to another pool's kernels it lost: at 4 requests/s of
+            # the 80/20 2D/3D stream, 2D p50 1.64 s with it against 1.28 s
+            # with the z and x launches apart (same box).  So the server's
+            # pools keep the launches apart unless SCFT_FILM says otherwise
+            # (the library reads it when a batch is created).
+            unset = "SCFT_FILM" not in os.environ
+            if unset:
+                os.environ["SCFT_FILM"] = "0"
+            try:
+                pool = Pool(self.lib, req.params, precision=self.precision,
+                            max_slots=self.slots.get(key, self.default_slots), max_nx=max(self.max_nx, nx),
+                            hbm_fraction=self.hbm_fraction, tol=self.tol, max_iters=self.max_iters)
+            finally:
+                if unset:
+                    os.environ.pop("SCFT_FILM", None)
             self.pools[key] = pool
         return pool
 
