@@ -682,6 +682,39 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
     }
     // D layout: register r of lane l is row l/16 + 4r, output point l%16
     // of the column tile; the mirrored half goes to n-1-k
+    if constexpr (sizeof(Tout) == 4 && HSP == 64) {
+        if (n == 2 * HSP && nr == 16) {
+            // FP32 rows of 128 points: the tile's results go through the
+            // wave's staging rows (16 rows x 512 B, the 16-byte chunks of row
+            // rr permuted by 4 (rr & 3), so that the four rows one write
+            // phase touches fall into different banks) and out as whole rows,
+            // 16 bytes per lane, instead of 4-byte stores of 64-byte runs
+            float *Yw = reinterpret_cast<float *>(Xw);
+            wave_sync();   // every lane has read its fragments from the staging rows
+#pragma unroll
+            for (int kt = 0; kt < NK; ++kt) {
+                const int k = 16 * kt + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int rr = lj + 4 * r, f = 4 * (rr & 3);
+                    const double u = U[kt][r], v = W[kt][r];
+                    const int k2 = n - 1 - k;
+                    Yw[rr * 128 + ((((k >> 2) ^ f)) << 2) + (k & 3)] = Tout(u + v);
+                    Yw[rr * 128 + ((((k2 >> 2) ^ f)) << 2) + (k2 & 3)] = Tout(u - v);
+                }
+            }
+            wave_sync();
+            typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                const int rr = 2 * it + (lane >> 5), c = lane & 31;
+                const f4 x = *reinterpret_cast<const f4 *>(Yw + rr * 128 + ((c ^ (4 * (rr & 3))) << 2));
+                *(__attribute__((address_space(1))) f4 *)((char *)dst +
+                                                          (unsigned(rbase + rr) * unsigned(n) + unsigned(4 * c)) * 4u) = x;
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int kt = 0; kt < NK; ++kt) {
         const int k = 16 * kt + li;
@@ -704,7 +737,7 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
     constexpr int LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
     __shared__ double Ms[HSP * LDM];
     __shared__ double Ma[HSP * LDM];
-    __shared__ double X[kMfma2Waves * 8 * XS];
+    __shared__ __attribute__((aligned(16))) double X[kMfma2Waves * 8 * XS];
     mfma2_stage_ops<HSP>(ops[0], Ms, Ma, threadIdx.x, 64 * kMfma2Waves);
     __syncthreads();
     const int lane = threadIdx.x & 63;
