@@ -43,7 +43,7 @@ METRIC = "converged SCFT solves/hour + p50/p99 time-to-converge at |\u0394F|\u22
 PEAK_HBM_GBS = 8000.0
 CONV_PROFILE = os.path.join(ROOT, "profiles", "c2_convergence_%s.json")   # per precision (fp32, fp32_nearest, fp64)
 # HBM bytes per z-operator dispatch from the FETCH_SIZE / WRITE_SIZE runs
-# (calibrated with bin/copy_probe); see profiles/c2_served_b192_hbm_traffic.md
+# (calibrated with a copy probe of known size); see profiles/c2_served_b192_hbm_traffic.md
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c2_served_b192_hbm_traffic.json")
 
 

@@ -682,13 +682,15 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
     }
     // D layout: register r of lane l is row l/16 + 4r, output point l%16
     // of the column tile; the mirrored half goes to n-1-k
-    if constexpr (sizeof(Tout) == 4 && HSP == 64) {
+    if constexpr (sizeof(Tout) == 4 && (HSP == 64 || HSP == 32)) {
         if (n == 2 * HSP && nr == 16) {
-            // FP32 rows of 128 points: the tile's results go through the
-            // wave's staging rows (16 rows x 512 B, the 16-byte chunks of row
-            // rr permuted by 4 (rr & 3), so that the four rows one write
+            // FP32 rows of n = 2 HSP points: the tile's results go through the
+            // wave's staging rows (16 rows of n floats, the 16-byte chunks of
+            // row rr permuted by 4 (rr & 3), so that the four rows one write
             // phase touches fall into different banks) and out as whole rows,
             // 16 bytes per lane, instead of 4-byte stores of 64-byte runs
+            constexpr int RC = 2 * HSP / 4;            // 16-byte chunks per row
+            constexpr int RPI = 64 / RC;               // rows per store instruction
             float *Yw = reinterpret_cast<float *>(Xw);
             wave_sync();   // every lane has read its fragments from the staging rows
 #pragma unroll
@@ -698,17 +700,17 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
                 for (int r = 0; r < 4; ++r) {
                     const int rr = lj + 4 * r, f = 4 * (rr & 3);
                     const double u = U[kt][r], v = W[kt][r];
-                    const int k2 = n - 1 - k;
-                    Yw[rr * 128 + ((((k >> 2) ^ f)) << 2) + (k & 3)] = Tout(u + v);
-                    Yw[rr * 128 + ((((k2 >> 2) ^ f)) << 2) + (k2 & 3)] = Tout(u - v);
+                    const int k2 = 2 * HSP - 1 - k;
+                    Yw[rr * 2 * HSP + ((((k >> 2) ^ f)) << 2) + (k & 3)] = Tout(u + v);
+                    Yw[rr * 2 * HSP + ((((k2 >> 2) ^ f)) << 2) + (k2 & 3)] = Tout(u - v);
                 }
             }
             wave_sync();
             typedef float f4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-            for (int it = 0; it < 8; ++it) {
-                const int rr = 2 * it + (lane >> 5), c = lane & 31;
-                const f4 x = *reinterpret_cast<const f4 *>(Yw + rr * 128 + ((c ^ (4 * (rr & 3))) << 2));
+            for (int it = 0; it < 16 / RPI; ++it) {
+                const int rr = RPI * it + lane / RC, c = lane % RC;
+                const f4 x = *reinterpret_cast<const f4 *>(Yw + rr * 2 * HSP + ((c ^ (4 * (rr & 3))) << 2));
                 *(__attribute__((address_space(1))) f4 *)((char *)dst +
                                                           (unsigned(rbase + rr) * unsigned(n) + unsigned(4 * c)) * 4u) = x;
             }
