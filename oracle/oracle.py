@@ -1,7 +1,8 @@
 """ctypes wrapper of liboracle.so, the CPU FP64 restatement of SCFT.c.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.
+bench.py's cpu_baseline leg, as the reference results are compared with or as
+the timed CPU baseline.
 The product never imports it.
 """
 import ctypes as C

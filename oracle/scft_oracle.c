@@ -2,7 +2,7 @@
  * scft_oracle.c — CPU FP64 restatement of the reference SCFT.c
  * (gahme-glitch/stimuli-responsive-nanodroplets).
  *
- * TEST INFRASTRUCTURE ONLY: the checker that libscft is compared with, and
+ * TEST INFRASTRUCTURE ONLY: the reference that libscft is compared with, and
  * the CPU baseline that bench.py times.  The product never links it.
  *
  * It follows the reference's algorithm literally (complex work arrays that

@@ -2,7 +2,8 @@
  * scft_oracle.h — CPU FP64 restatement of SCFT.c, TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * may load this code, and only as the checker or the timed CPU baseline.
+ * may load this code, and only as the reference results are compared with
+ * or as the timed CPU baseline.
  * The product (libscft) never links or calls it.
  */
 #ifndef SCFT_ORACLE_H
