@@ -164,7 +164,7 @@ constexpr int kE2fMinV = 16384;
 // Process-wide autotuner results: (n, dirichlet, alpha bits, storage,
 // slots timed, NX, candidate set) -> candidate (autotune_z).
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, long long, int, int, int, int>, int> g_tune;
+std::map<std::tuple<int, int, long long, int, int, int, int, int, int>, int> g_tune;
 
 // The autotuner times as many slots as put this many voxels in one launch
 // (at most the batch): one small slot alone times launch latency only.
@@ -257,7 +257,8 @@ struct scft_ctx {
     bool fft_reg = true;             // register FFTs for periodic lengths 64, 96, 128, 192, 256, 320
     bool fft_reg_force = false;      // register FFTs also for launches of few lines
     int fftx_mode = 2;               // periodic lines through LDS in 32-line groups (k_fftx):
-                                     // 0 never, 1 every eligible length, 2 length 128 in small launches
+                                     // 0 never, 1 every eligible length, 2 autotuned per size class
+    bool xcoal_cls[3] = {};          // k_fftx chosen for active sets of <= 16, <= 64, more slots
     bool dense_mfma = true;          // MFMA dense z operator when hs <= 64
     bool dense_mfma1 = false;        // force the one-wave-per-SIMD MFMA kernel for even NZ too
     bool tmp_wide = false;           // FP64 scratch between axis operators in FP32 mode
@@ -282,6 +283,7 @@ struct scft_ctx {
     int film_mode = -1;              // fused z + x step on 2D films: 0 off, 1 on, -1 autotuned (SCFT_FILM)
     bool film_cls[3] = {};           // film step chosen for active sets of <= 16, <= 64, more slots
     int nstep0 = 0;                  // launches of contour step 0
+    std::vector<int> step_first;     // index of each contour step's first launch
     bool inflight = false;           // a batched iteration launched by scft_batch_launch, not yet completed
     PrepArgs *d_prep = nullptr;
     DenArgs *d_den = nullptr;
@@ -715,6 +717,9 @@ unsigned round_key(const scft_ctx *c, int ci, int t, int ai, bool last)
 // x lengths of the fused film step (the register-FFT forms of 16 lanes)
 bool film_length(int L) { return L == 192 || L == 256 || L == 320; }
 
+// periodic lengths of k_fftx (32-line LDS groups)
+bool fftx_length(int L) { return L == 128 || L == 192 || L == 256 || L == 320; }
+
 // The film step against the split step is tuned for three sizes of the
 // active set: it wins where few chains share a launch (its fused launch
 // halves the launches and keeps the scratch on chip) and can lose on a full
@@ -737,6 +742,7 @@ int build_schedule(scft_ctx *c)
         if (c->slots[i].ready && c->slots[i].active) c->act.push_back(i);
     c->launches.clear();
     c->nstep0 = 0;
+    c->step_first.clear();
     std::vector<LineOp> lops;
     std::vector<DenseOp> dops;
     std::vector<PlaneOp> pops;
@@ -759,6 +765,7 @@ int build_schedule(scft_ctx *c)
         if (std::find(film_lens.begin(), film_lens.end(), s.NX) == film_lens.end()) film_lens.push_back(s.NX);
     }
     for (int t = 0; t < maxsteps; ++t) {
+        c->step_first.push_back(int(c->launches.size()));
         if (film) {
             for (int L : film_lens) {
                 Launch ln{};
@@ -963,12 +970,11 @@ int build_schedule(scft_ctx *c)
                 // wave-per-line FFT gives every line pair its own wave there.
                 const bool few = !c->fft_reg_force && c->fft_wave && fftwave_supported(L) && units < 2 * c->ncu;
                 // 32 lines that start at 32 consecutive elements, stored (last axis): k_fftx
-                // (measured on C3, 128 x 128 x 64: 4.5% faster per iteration than the 8 x 16
-                // register FFT with one solve, 4.6% slower with four; the same as the 16-lane
-                // register FFTs for 192, 256 and 320 on C2 and C4)
-                bool coal = !c->tmp_wide && !dirichlet &&
-                            (c->fftx_mode == 1 ? (L == 128 || L == 192 || L == 256 || L == 320)
-                                               : (c->fftx_mode == 2 && L == 128 && units <= 12 * c->ncu));
+                // where the autotuner measured it faster for the active set's size class
+                // (on C3, 128 x 128 x 64, 4.5% faster per iteration than the 8 x 16 register
+                // FFT with one solve, 4.6% slower with four)
+                bool coal = !c->tmp_wide && !dirichlet && fftx_length(L) &&
+                            (c->fftx_mode == 1 || (c->fftx_mode == 2 && c->xcoal_cls[film_class(int(c->act.size()))]));
                 for (int k = ln.first; coal && k < ln.first + ln.count; ++k) {
                     const LineOp &op = lops[k];
                     coal = !op.epre && !op.epost && op.s0 == 1 && op.c1 % 32 == 0 && op.s1 % 32 == 0 &&
@@ -1341,9 +1347,12 @@ void free_ctx(scft_ctx *c)
     delete c;
 }
 
-// Autotune the z operator for this context's size: time the z launch of
+// Autotune the contour step for this context's size: time the launches of
 // the first step over every chain of a batch-sized set of slots with each
-// candidate and keep the faster.
+// z candidate and keep the faster; then, for each size class of the active
+// set, the fused film step against the split step, and the periodic
+// last-axis transform through LDS groups (k_fftx) against the register or
+// wave-per-line FFT.
 int autotune_z(scft_ctx *c)
 {
     const scft_params &p = c->proto;
@@ -1374,27 +1383,37 @@ int autotune_z(scft_ctx *c)
         if (!(film_ok && c->film_mode == 1)) cands.push_back(1);
         if (film_ok) cands.push_back(2);
     }
-    // result: bits 0-1 the candidate at nt slots, bit 2 + k the film step for size class k
+    // the last-axis form is tuned where k_fftx takes a periodic length of the grid
+    // (SCFT_XFFT unset) and the scratch between axes is the slice type
+    const bool xtune = c->fftx_mode == 2 && !c->tmp_wide &&
+                       ((d0.nx > 1 && fftx_length(d0.nx)) || (c->NY > 1 && fftx_length(c->NY)));
+    // result: bits 0-1 the candidate at nt slots, bit 2 + k the film step for
+    // size class k, bit 5 + k k_fftx for size class k
     auto apply = [&](int r) {
         const int k = r & 3;
         c->zkind = k == 0 ? Z_FFT : Z_DENSE;
         for (int i = 0; i < 3; ++i) c->film_cls[i] = k != 0 && ((r >> (2 + i)) & 1);
+        for (int i = 0; i < 3; ++i) c->xcoal_cls[i] = (r >> (5 + i)) & 1;
     };
-    if (cands.size() == 1) {
+    if (cands.size() == 1 && !xtune) {
         apply(cands[0] == 2 ? (2 | (7 << 2)) : cands[0]);
         return SCFT_OK;
     }
-    auto key = std::make_tuple(c->NZ, p.mask ? 1 : 0, abits, int(c->tsize), nt, d0.nx, int(cands.size()) * 4 + cands[0]);
+    auto key = std::make_tuple(c->NZ, p.mask ? 1 : 0, abits, int(c->tsize), nt, d0.nx,
+                               int(cands.size()) * 4 + cands[0], c->NY, xtune ? 1 : 0);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
         if (it != g_tune.end()) { apply(it->second); return SCFT_OK; }
     }
-    // mean time of the launches of contour step 0 with `ns` slots active (the
-    // x launch is the same for the z kinds unless a fused step replaces both axes)
-    auto time_step0 = [&](int kind, int ns, double *ms_out) -> int {
+    // time of the launches of contour step 0 and of the middle step with `ns`
+    // slots active; SCFT_TUNE_LOG=1: every timing to stderr
+    const char *tl = std::getenv("SCFT_TUNE_LOG");
+    const bool tune_log = tl && tl[0] == '1';
+    auto time_step0 = [&](int kind, int ns, double *ms_out, bool xcoal = false) -> int {
         c->zkind = kind == 0 ? Z_FFT : Z_DENSE;
         for (int i = 0; i < 3; ++i) c->film_cls[i] = kind == 2;
+        for (int i = 0; i < 3; ++i) c->xcoal_cls[i] = xcoal;
         for (int t = 0; t < c->nslots; ++t) {
             if (t < ns) {
                 RETCHK(slot_init(c, c->slots[t], c->proto));
@@ -1404,22 +1423,52 @@ int autotune_z(scft_ctx *c)
             }
         }
         RETCHK(build_schedule(c));
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
+        // the launches of contour step 0 (every chain) and of the middle step
+        // (the chains that run the whole contour)
         std::vector<Launch> zl(c->launches.begin(), c->launches.begin() + c->nstep0);
+        const int nst = int(c->step_first.size());
+        if (nst > 2) {
+            const int a = c->step_first[nst / 2], b = c->step_first[nst / 2 + 1];
+            zl.insert(zl.end(), c->launches.begin() + a, c->launches.begin() + b);
+        }
         std::swap(zl, c->launches);
+        // 20 repetitions of those launches as one graph, as the iteration
+        // runs them: eager launches of small kernels would time the host
         const bool f32 = c->tsize == 4;
-        for (int w = 0; w < 3; ++w) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
-        HIPCHK(hipEventRecord(e0, c->stream));
-        for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
-        HIPCHK(hipEventRecord(e1, c->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        hipGraph_t g = nullptr;
+        hipGraphExec_t gx = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
+            e = hipStreamEndCapture(c->stream, &g);
+        }
         std::swap(zl, c->launches);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        if (e == hipSuccess) e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = hipGraphLaunch(gx, c->stream);   // warm
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {   // the fastest of three
+            float m = 0.f;
+            e = hipEventRecord(e0, c->stream);
+            if (e == hipSuccess) e = hipGraphLaunch(gx, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+            ms = std::min(ms, m);
+        }
+        if (e1) (void)hipEventDestroy(e1);
+        if (e0) (void)hipEventDestroy(e0);
+        if (gx) (void)hipGraphExecDestroy(gx);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            set_err(std::string("autotune timing: ") + hipGetErrorString(e));
+            return SCFT_EHIP;
+        }
+        if (tune_log)
+            std::fprintf(stderr, "scft autotune: NX %d NY %d NZ %d, %d slots, z form %d, k_fftx %d: %.2f us per first "
+                         "and middle step\n", d0.nx, c->NY, c->NZ, ns, kind, xcoal ? 1 : 0, 1000.0 * ms / 20.0);
         *ms_out = ms;
         return SCFT_OK;
     };
@@ -1429,12 +1478,14 @@ int autotune_z(scft_ctx *c)
     int choice = cands[0];
     double best = 1e30;
     for (int kind : cands) {
+        if (cands.size() == 1) break;
         double ms;
         RETCHK(time_step0(kind, nt, &ms));
         if (kind == 2) ms /= kFilmMargin;
         if (ms < best) { best = ms; choice = kind; }
     }
     int result = choice | (choice == 2 ? (4 << 2) : 0);
+    if (cands.size() == 1 && choice == 2) result |= 7 << 2;   // the film step forced
     // the film step for the smaller size classes, timed against the split step
     const bool has_film = std::find(cands.begin(), cands.end(), 2) != cands.end();
     const bool has_split = std::find(cands.begin(), cands.end(), 1) != cands.end();
@@ -1449,6 +1500,25 @@ int autotune_z(scft_ctx *c)
             f = t2 < kFilmMargin * t1;
         }
         if (f) result |= 1 << (2 + k);
+    }
+    // k_fftx against the rule's last-axis form, per size class of the split
+    // step (the film step has its own x transform); like the film step it
+    // must win by the margin
+    if (xtune) {
+        std::map<int, bool> at;   // slots timed -> k_fftx chosen
+        for (int k = 0; k < 3; ++k) {
+            if ((result >> (2 + k)) & 1) continue;
+            const int ns = std::min(kFilmTuneSlots[k], nt);
+            auto it = at.find(ns);
+            if (it == at.end()) {
+                const int kind = choice == 0 ? 0 : 1;
+                double t1, t2;
+                RETCHK(time_step0(kind, ns, &t1, false));
+                RETCHK(time_step0(kind, ns, &t2, true));
+                it = at.emplace(ns, t2 < kFilmMargin * t1).first;
+            }
+            if (it->second) result |= 1 << (5 + k);
+        }
     }
     for (int t = 0; t < c->nslots; ++t) {
         c->slots[t].active = false;
