@@ -190,9 +190,20 @@ class Pool:
     def _admit(self):
         while self.queue and self.free_slots:
             req = self.queue[0]
-            if req.seed[0] == SEED_FROM and self.by_id[req.seed[1]].status in ("queued", "running"):
-                # its seed has not converged yet: keep order, try later
-                break
+            if req.seed[0] == SEED_FROM:
+                src = self.by_id.get(req.seed[1])
+                if src is not None and src.status in ("queued", "running"):
+                    # its seed has not converged yet: keep order, try later
+                    break
+                if src is None or src.fields is None:
+                    # the seed request ended without converged fields kept
+                    # (failed, out of budget, keep_fields off, or unknown):
+                    # this request cannot start; it fails without a slot
+                    self.queue.popleft()
+                    req.status = "seed_failed"
+                    req.done_t = time.perf_counter()
+                    self.done.append(req)
+                    continue
             self.queue.popleft()
             s = self.free_slots.popleft()
             self.batch.init_slot(s, req.params)

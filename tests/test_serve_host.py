@@ -75,3 +75,30 @@ def test_advance_lets_each_pool_run_at_its_own_pace():
         srv.advance()
     assert (fast.completed, slow.completed) == (50, 10)
     assert not fast.inflight and not slow.inflight
+
+
+def test_seed_from_a_request_without_fields_fails_without_a_slot():
+    """A SEED_FROM request whose seed request ended without kept fields
+    (failed, or keep_fields off) is marked seed_failed and takes no slot;
+    one whose seed is still running waits at the queue head."""
+    import collections
+    import serve
+    pool = serve.Pool.__new__(serve.Pool)
+    pool.queue = collections.deque()
+    pool.free_slots = collections.deque([0, 1])
+    pool.by_id = {}
+    pool.done = []
+    pool.running = {}
+    src_failed = serve.Request("a", None, (serve.SEED_DROPLET, None))
+    src_failed.status = "failed"
+    src_running = serve.Request("b", None, (serve.SEED_DROPLET, None))
+    src_running.status = "running"
+    pool.by_id = {"a": src_failed, "b": src_running}
+    r1 = serve.Request("t1", None, (serve.SEED_FROM, "a"))
+    r2 = serve.Request("t2", None, (serve.SEED_FROM, "b"))
+    pool.queue.extend([r1, r2])
+    pool.by_id.update({"t1": r1, "t2": r2})
+    pool._admit()
+    assert r1.status == "seed_failed" and r1 in pool.done and r1.slot is None
+    assert list(pool.queue) == [r2] and r2.status == "queued"
+    assert list(pool.free_slots) == [0, 1] and not pool.running
