@@ -229,6 +229,7 @@ struct Launch {
     int hsp = 0;     // padded dense half width (MFMA kind)
     size_t lds;
     FilmTab ftab{};  // tables of a fused film step (LK_FILM)
+    int step = 0;    // contour step
 };
 
 bool fftwave_supported(int L)
@@ -248,6 +249,11 @@ struct scft_ctx {
     int nslots = 0, max_nx = 0, max_V = 0;
     int NY = 0, NZ = 0;
     hipStream_t stream = nullptr;
+    // launches of one step and axis position (one per x length) run side by side
+    // on these streams, forked from and joined back into the context's stream
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> evs;
+    bool par_axes = true;            // SCFT_PARX=0: one after another
     std::vector<Slot> slots;
     std::vector<void *> allocs;
     std::vector<std::unique_ptr<FftTable>> ffts;
@@ -1010,6 +1016,10 @@ int build_schedule(scft_ctx *c)
         }
         if (t == 0) c->nstep0 = int(c->launches.size());
     }
+    for (size_t t = 0; t < c->step_first.size(); ++t) {
+        const size_t e = t + 1 < c->step_first.size() ? size_t(c->step_first[t + 1]) : c->launches.size();
+        for (size_t i = size_t(c->step_first[t]); i < e; ++i) c->launches[i].step = int(t);
+    }
     RETCHK(upload(c, &c->lineops, &c->lineops_cap, lops));
     RETCHK(upload(c, &c->denseops, &c->denseops_cap, dops));
     RETCHK(upload(c, &c->planeops, &c->planeops_cap, pops));
@@ -1097,11 +1107,10 @@ void launch_prep(scft_ctx *c, int nact, int maxV)
 // Typed launchers: slices are T; the scratch between axes is T too, or FP64
 // when tmp_wide (then the axis position fixes the input/output types).
 template <typename Tin, typename Tout>
-void launch_fftwave(scft_ctx *c, const Launch &ln)
+void launch_fftwave(scft_ctx *c, const Launch &ln, hipStream_t st)
 {
     const LineOp *ops = c->lineops + ln.first;
     dim3 g(ln.gx, ln.count), b(64 * kFftWaves);
-    hipStream_t st = c->stream;
     switch (ln.L) {
 #define SCFT_FFTW_CASE(LL) \
     case LL: hipLaunchKernelGGL((k_fftwave<LL, Tin, Tout>), g, b, 0, st, ops); break;
@@ -1114,10 +1123,9 @@ void launch_fftwave(scft_ctx *c, const Launch &ln)
 }
 
 template <typename Tin, typename Tout>
-void launch_one(scft_ctx *c, const Launch &ln)
+void launch_one(scft_ctx *c, const Launch &ln, hipStream_t st)
 {
     dim3 g(ln.gx, ln.count);
-    hipStream_t st = c->stream;
     switch (ln.kind) {
     case LK_PLANE:
         hipLaunchKernelGGL((k_plane64<Tin, Tout>), dim3(ln.gx), dim3(64 * kPlaneWaves), 0, st,
@@ -1156,7 +1164,7 @@ void launch_one(scft_ctx *c, const Launch &ln)
         hipLaunchKernelGGL((k_zdense<Tin, Tout>), g, dim3(kThreads), 0, st, c->denseops + ln.first);
         break;
     case LK_FFTWAVE:
-        launch_fftwave<Tin, Tout>(c, ln);
+        launch_fftwave<Tin, Tout>(c, ln, st);
         break;
     case LK_FFTREG2:
         if (ln.L == 320)
@@ -1196,20 +1204,65 @@ void launch_one(scft_ctx *c, const Launch &ln)
 }
 
 template <typename T>
-void launch_steps(scft_ctx *c)
+void launch_typed(scft_ctx *c, const Launch &ln, hipStream_t st)
 {
-    for (const Launch &ln : c->launches) {
-        if (!c->tmp_wide) {
-            launch_one<T, T>(c, ln);
+    if (!c->tmp_wide) {
+        launch_one<T, T>(c, ln, st);
+        return;
+    }
+    switch (ln.pos) {
+    case 0: launch_one<T, double>(c, ln, st); break;
+    case 1: launch_one<double, double>(c, ln, st); break;
+    case 2: launch_one<double, T>(c, ln, st); break;
+    default: launch_one<T, T>(c, ln, st); break;
+    }
+}
+
+// The schedule's launches in order.  With `par`, the launches of one step
+// and axis position (a batch of uneven NX has one per x length; they touch
+// different chains) run side by side: the first on the context's stream,
+// the others on side streams that wait for the work before them and that
+// the context's stream waits for before the next launch.  Without `par`
+// (timing single launches) they run one after another.
+template <typename T>
+int launch_steps(scft_ctx *c, bool par = false)
+{
+    const std::vector<Launch> &L = c->launches;
+    const size_t n = L.size();
+    size_t nev = 0;
+    auto next_event = [&](hipEvent_t *e) -> int {
+        if (nev == c->evs.size()) {
+            hipEvent_t x = nullptr;
+            HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+            c->evs.push_back(x);
+        }
+        *e = c->evs[nev++];
+        return SCFT_OK;
+    };
+    for (size_t i = 0; i < n;) {
+        size_t j = i + 1;
+        if (par && c->par_axes)
+            while (j < n && j - i <= c->side.size() && L[j].step == L[i].step && L[j].pos == L[i].pos) ++j;
+        if (j - i == 1) {
+            launch_typed<T>(c, L[i], c->stream);
+            i = j;
             continue;
         }
-        switch (ln.pos) {
-        case 0: launch_one<T, double>(c, ln); break;
-        case 1: launch_one<double, double>(c, ln); break;
-        case 2: launch_one<double, T>(c, ln); break;
-        default: launch_one<T, T>(c, ln); break;
+        hipEvent_t fork;
+        RETCHK(next_event(&fork));
+        HIPCHK(hipEventRecord(fork, c->stream));
+        for (size_t k = 1; k < j - i; ++k) HIPCHK(hipStreamWaitEvent(c->side[k - 1], fork, 0));
+        launch_typed<T>(c, L[i], c->stream);
+        for (size_t k = 1; k < j - i; ++k) launch_typed<T>(c, L[i + k], c->side[k - 1]);
+        for (size_t k = 1; k < j - i; ++k) {
+            hipEvent_t join;
+            RETCHK(next_event(&join));
+            HIPCHK(hipEventRecord(join, c->side[k - 1]));
+            HIPCHK(hipStreamWaitEvent(c->stream, join, 0));
         }
+        i = j;
     }
+    return SCFT_OK;
 }
 
 template <typename T>
@@ -1257,8 +1310,13 @@ int enqueue(scft_ctx *c, int phases)
     if (nact == 0) return SCFT_OK;
     const bool f32 = c->tsize == 4;
     if (phases & 1) {
-        if (f32) { launch_prep<float>(c, nact, maxV_active(c)); launch_steps<float>(c); }
-        else { launch_prep<double>(c, nact, maxV_active(c)); launch_steps<double>(c); }
+        if (f32) {
+            launch_prep<float>(c, nact, maxV_active(c));
+            RETCHK(launch_steps<float>(c, true));
+        } else {
+            launch_prep<double>(c, nact, maxV_active(c));
+            RETCHK(launch_steps<double>(c, true));
+        }
     }
     if (phases & 2) {
         if (f32) launch_densities<float>(c, nact);
@@ -1343,6 +1401,8 @@ void free_ctx(scft_ctx *c)
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
     if (c->rng_host) (void)hipHostFree(c->rng_host);
+    for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+    for (hipStream_t st : c->side) (void)hipStreamDestroy(st);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1440,7 +1500,7 @@ int autotune_z(scft_ctx *c)
         hipEvent_t e0 = nullptr, e1 = nullptr;
         hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c) : launch_steps<double>(c);
+            for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c, true) : launch_steps<double>(c, true);
             e = hipStreamEndCapture(c->stream, &g);
         }
         std::swap(zl, c->launches);
@@ -1623,6 +1683,18 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_err("hipStreamCreate failed");
         return fail(SCFT_EHIP);
+    }
+    // side streams for the launches of one step and axis (one per x length): a
+    // batch of uneven NX has up to three lengths of the register FFTs
+    const char *px = std::getenv("SCFT_PARX");
+    c->par_axes = !(px && px[0] == '0');
+    for (int k = 0; c->par_axes && k < 3; ++k) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            set_err("hipStreamCreate failed");
+            return fail(SCFT_EHIP);
+        }
+        c->side.push_back(st);
     }
     // the FFT tiles may use more than 64 KB of LDS on gfx950 (160 KB per workgroup)
     const void *lk[] = {(const void *)k_lineop<double, double>, (const void *)k_lineop<float, double>,
