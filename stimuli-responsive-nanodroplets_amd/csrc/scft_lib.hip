@@ -1020,6 +1020,25 @@ int build_schedule(scft_ctx *c)
         const size_t e = t + 1 < c->step_first.size() ? size_t(c->step_first[t + 1]) : c->launches.size();
         for (size_t i = size_t(c->step_first[t]); i < e; ++i) c->launches[i].step = int(t);
     }
+    // side streams for launches of one step and axis position (one per x length
+    // of a batch of uneven NX, up to three), created when a schedule first has
+    // such launches: every stream of the process shares its few hardware queues
+    if (c->par_axes && c->side.size() < 3) {
+        size_t need = 0;
+        for (size_t i = 0; i < c->launches.size();) {
+            size_t j = i + 1;
+            while (j < c->launches.size() && c->launches[j].step == c->launches[i].step &&
+                   c->launches[j].pos == c->launches[i].pos)
+                ++j;
+            need = std::max(need, std::min<size_t>(j - i - 1, 3));
+            i = j;
+        }
+        while (c->side.size() < need) {
+            hipStream_t st = nullptr;
+            HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            c->side.push_back(st);
+        }
+    }
     RETCHK(upload(c, &c->lineops, &c->lineops_cap, lops));
     RETCHK(upload(c, &c->denseops, &c->denseops_cap, dops));
     RETCHK(upload(c, &c->planeops, &c->planeops_cap, pops));
@@ -1684,18 +1703,8 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
         set_err("hipStreamCreate failed");
         return fail(SCFT_EHIP);
     }
-    // side streams for the launches of one step and axis (one per x length): a
-    // batch of uneven NX has up to three lengths of the register FFTs
     const char *px = std::getenv("SCFT_PARX");
     c->par_axes = !(px && px[0] == '0');
-    for (int k = 0; c->par_axes && k < 3; ++k) {
-        hipStream_t st = nullptr;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-            set_err("hipStreamCreate failed");
-            return fail(SCFT_EHIP);
-        }
-        c->side.push_back(st);
-    }
     // the FFT tiles may use more than 64 KB of LDS on gfx950 (160 KB per workgroup)
     const void *lk[] = {(const void *)k_lineop<double, double>, (const void *)k_lineop<float, double>,
                         (const void *)k_lineop<double, float>, (const void *)k_lineop<float, float>};

@@ -498,23 +498,27 @@ class Server:
                 if other.inflight:
                     other.step_complete()
             nx = grid_of(req.params)[0]
-            # Pools of different structures run concurrently here.  The fused
-            # film step wins on an idle GPU where few chains share a launch,
-            # but next to another pool's kernels it lost: at 4 requests/s of
-            # the 80/20 2D/3D stream, 2D p50 1.64 s with it against 1.28 s
-            # with the z and x launches apart (same box).  So the server's
-            # pools keep the launches apart unless SCFT_FILM says otherwise
-            # (the library reads it when a batch is created).
-            unset = "SCFT_FILM" not in os.environ
-            if unset:
-                os.environ["SCFT_FILM"] = "0"
+            # Pools of different structures run concurrently here, each on
+            # its own stream, so the server's pools are created with two
+            # library defaults changed (unless the variables are set; the
+            # library reads them when a batch is created):
+            # - SCFT_FILM=0: the fused film step wins on an idle GPU where few
+            #   chains share a launch, but next to another pool's kernels it
+            #   lost: at 4 requests/s of the 80/20 2D/3D stream, 2D p50 1.64 s
+            #   with it against 1.28 s with the z and x launches apart;
+            # - SCFT_PARX=0: a pool of uneven NX would run its x launches side
+            #   by side on extra streams, which share the process's few
+            #   hardware queues with the other pools' streams: p50 3.37 /
+            #   8.39 s at 4 / 6 requests/s against 1.60 / 2.64 s without.
+            pins = {k: "0" for k in ("SCFT_FILM", "SCFT_PARX") if k not in os.environ}
+            os.environ.update(pins)
             try:
                 pool = Pool(self.lib, req.params, precision=self.precision,
                             max_slots=self.slots.get(key, self.default_slots), max_nx=max(self.max_nx, nx),
                             hbm_fraction=self.hbm_fraction, tol=self.tol, max_iters=self.max_iters)
             finally:
-                if unset:
-                    os.environ.pop("SCFT_FILM", None)
+                for k in pins:
+                    os.environ.pop(k, None)
             self.pools[key] = pool
         return pool
 
