@@ -65,6 +65,8 @@ def dominant_kernel(schedule):
     n = dict((k, int(v)) for k, v in (kv.split("=") for kv in schedule.split() if "=" in kv and kv[0] != "z"))
     if n.get("film", 0):
         return "k_film (dense z operator and x transform fused, one launch per contour step and x length)"
+    if n.get("mfma2_pf", 0):
+        return "k_zdense_mfma2p (z operator of every chain, one launch per contour step, next tile's slice rows in flight)"
     if n.get("dense_mfma2", 0):
         return "k_zdense_mfma2 (z operator of every chain, one launch per contour step)"
     return "first axis of every chain, one launch per contour step"

@@ -592,63 +592,90 @@ __device__ __forceinline__ void mfma2_stage_ops(const DenseOp &op, double *Ms, d
     }
 }
 
-// One 16-row tile (rows rbase..rbase+15 of op) of the z operator, result to
-// dst (the op's output, or a caller's scratch with the same layout).  Ms/Ma
-// are the staged operator halves, Xw the wave's 8-row staging rows.
-template <typename Tin, typename Tout, int HSP>
-__device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *dst, const double *Ms,
-                                           const double *Ma, double *Xw, int lane)
+// The loads of one 16-row tile: two points per lane from each row of the
+// slice and of the step's factor, held until the tile is staged.  An FP32
+// E^2 pair (mode 4) is held as its raw 8 bytes in ea[r].x and widened at
+// staging, so that every mode issues the same loads in the same order.
+template <typename Tin>
+struct Z2Regs {
+    Pair<Tin> xa[16];
+    Pair<double> ea[16];
+};
+
+// factor: E alone (start slice), E^2 inside a block (FP64 E squared, or
+// the FP32 E^2 array: mode 4), E E2 at a block boundary
+__device__ __forceinline__ int mfma2_mode(const DenseOp &op)
 {
-    constexpr int NT = Mfma2<HSP>::NT, NK = Mfma2<HSP>::NK, LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
-    const int n = op.n, hs = op.hs;   // n = 2 hs
-    const int li = lane & 15, lj = lane >> 4;
-    const int c = 2 * lane;                 // this lane's point pair in a row
-    const bool cl = c < 2 * HSP;            // lanes that own staged columns
+    return !op.epre2 ? 1 : (op.epre2 == op.epre ? (op.e2f ? 4 : 2) : 3);
+}
+
+// E2F = false: the caller has no FP32 E^2 factor (one load path).
+template <typename Tin, int HSP, bool E2F = true>
+__device__ __forceinline__ void mfma2_load(const DenseOp &op, int rbase, int lane, Z2Regs<Tin> &R)
+{
+    const int n = op.n;
+    const int c = 2 * lane;
     const int cc = c + 1 < n ? c : 0;       // load column (pairs past the row read column 0)
     const int nr = min(16, op.nrows - rbase);
     const Tin *src = static_cast<const Tin *>(op.src);
-    // factor: E alone (start slice), E^2 inside a block (FP64 E squared, or
-    // the FP32 E^2 array: mode 4), E E2 at a block boundary
-    const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? (op.e2f ? 4 : 2) : 3);
-    Pair<Tin> xa[16];
-    Pair<double> ea[16];
     const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(cc);
-    if (mode == 4) {
+    if (E2F && mfma2_mode(op) == 4) {
+        const double *e2 = reinterpret_cast<const double *>(op.e2f);   // a float pair per 8 bytes
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
-            xa[r] = ldg_pair_nt(src, a);   // the slice is read once; the factors are reused every step
-            const Pair<float> f = ldg_pair(op.e2f, a);
-            ea[r] = Pair<double>{double(f.x), double(f.y)};
+            R.xa[r] = ldg_pair_nt(src, a);   // the slice is read once; the factors are reused every step
+            R.ea[r] = Pair<double>{ldg(e2, a >> 1), 0.0};   // a is even
         }
     } else {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const unsigned a = o0 + unsigned(r < nr ? r : 0) * unsigned(n);
-            xa[r] = ldg_pair_nt(src, a);
-            ea[r] = ldg_pair(op.epre, a);
+            R.xa[r] = ldg_pair_nt(src, a);
+            R.ea[r] = ldg_pair(op.epre, a);
         }
     }
-    double sf[NT], af[NT];
+}
+
+// Stage a loaded tile into the wave's 8-row LDS rows with its factor
+// applied, in two halves, and read each lane's symmetric / antisymmetric
+// A fragments (sf, af) back.
+template <typename Tin, int HSP>
+__device__ __forceinline__ void mfma2_stage(const DenseOp &op, int rbase, const Z2Regs<Tin> &R, double *Xw, int lane,
+                                            double (&sf)[HSP / 4], double (&af)[HSP / 4])
+{
+    constexpr int NT = Mfma2<HSP>::NT, XS = Mfma2<HSP>::XS;
+    const int n = op.n;
+    const int li = lane & 15, lj = lane >> 4;
+    const int c = 2 * lane;                 // this lane's point pair in a row
+    const bool cl = c < 2 * HSP;            // lanes that own staged columns
+    const int cc = c + 1 < n ? c : 0;
+    const int nr = min(16, op.nrows - rbase);
+    const int mode = mfma2_mode(op);
+    const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(cc);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         wave_sync();   // the previous reads of the staging rows are done
         if (cl) {
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const int R = 8 * h + r;
-                double e0 = ea[R].x, e1 = ea[R].y;
-                if (mode == 2) {
+                const int Rr = 8 * h + r;
+                double e0 = R.ea[Rr].x, e1 = R.ea[Rr].y;
+                if (mode == 4) {
+                    const long long bits = __double_as_longlong(R.ea[Rr].x);
+                    e0 = double(__int_as_float(int(bits & 0xFFFFFFFFll)));
+                    e1 = double(__int_as_float(int(bits >> 32)));
+                } else if (mode == 2) {
                     e0 *= e0;
                     e1 *= e1;
                 } else if (mode == 3) {   // one step per block: load the second factor here
-                    const Pair<double> f = ldg_pair(op.epre2, o0 + unsigned(R < nr ? R : 0) * unsigned(n));
+                    const Pair<double> f = ldg_pair(op.epre2, o0 + unsigned(Rr < nr ? Rr : 0) * unsigned(n));
                     e0 *= f.x;
                     e1 *= f.y;
                 }
                 double2 v;
-                v.x = double(xa[R].x) * e0;
-                v.y = double(xa[R].y) * e1;
+                v.x = double(R.xa[Rr].x) * e0;
+                v.y = double(R.xa[Rr].y) * e1;
                 *reinterpret_cast<double2 *>(&Xw[r * XS + c]) = v;
             }
         }
@@ -664,6 +691,22 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
             }
         }
     }
+}
+
+// The products of a staged tile and its stores to dst (the op's output,
+// or a caller's scratch with the same layout).  Ms/Ma are the staged
+// operator halves, Xw the wave's staging rows (reused for the FP32 rows).
+// ROWS: the caller guarantees n = 2 HSP and whole 16-row tiles (FP32
+// output), so the row-store path is the only one compiled.
+template <typename Tout, int HSP, bool ROWS = false>
+__device__ __forceinline__ void mfma2_mma_store(const DenseOp &op, int rbase, Tout *dst, const double *Ms,
+                                                const double *Ma, double *Xw, int lane, const double (&sf)[HSP / 4],
+                                                const double (&af)[HSP / 4])
+{
+    constexpr int NT = Mfma2<HSP>::NT, NK = Mfma2<HSP>::NK, LDM = Mfma2<HSP>::LDM;
+    const int n = op.n, hs = op.hs;   // n = 2 hs
+    const int li = lane & 15, lj = lane >> 4;
+    const int nr = min(16, op.nrows - rbase);
     d4_t U[NK], W[NK];
 #pragma unroll
     for (int kt = 0; kt < NK; ++kt) {
@@ -682,8 +725,9 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
     }
     // D layout: register r of lane l is row l/16 + 4r, output point l%16
     // of the column tile; the mirrored half goes to n-1-k
+    static_assert(!ROWS || (sizeof(Tout) == 4 && (HSP == 64 || HSP == 32)), "row stores: FP32 rows of 64 or 128");
     if constexpr (sizeof(Tout) == 4 && (HSP == 64 || HSP == 32)) {
-        if (n == 2 * HSP && nr == 16) {
+        if (ROWS || (n == 2 * HSP && nr == 16)) {
             // FP32 rows of n = 2 HSP points: the tile's results go through the
             // wave's staging rows (16 rows of n floats, the 16-byte chunks of
             // row rr permuted by 4 (rr & 3), so that the four rows one write
@@ -733,6 +777,19 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
     }
 }
 
+// One 16-row tile (rows rbase..rbase+15 of op) of the z operator, result to
+// dst: loads, staging, products and stores in sequence.
+template <typename Tin, typename Tout, int HSP>
+__device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *dst, const double *Ms,
+                                           const double *Ma, double *Xw, int lane)
+{
+    Z2Regs<Tin> R;
+    mfma2_load<Tin, HSP>(op, rbase, lane, R);
+    double sf[HSP / 4], af[HSP / 4];
+    mfma2_stage<Tin, HSP>(op, rbase, R, Xw, lane, sf, af);
+    mfma2_mma_store<Tout, HSP>(op, rbase, dst, Ms, Ma, Xw, lane, sf, af);
+}
+
 template <typename Tin, typename Tout, int HSP>
 __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp *__restrict__ ops, int ntiles, int tpo)
 {
@@ -748,6 +805,57 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
     for (int t = blockIdx.x * kMfma2Waves + wave; t < ntiles; t += gridDim.x * kMfma2Waves) {
         const DenseOp &op = ops[t / tpo];
         mfma2_tile<Tin, Tout, HSP>(op, (t % tpo) * 16, static_cast<Tout *>(op.dst), Ms, Ma, Xw, lane);
+    }
+}
+
+// The same with the next tile's loads in flight under each tile's products
+// (issued once the tile is staged, into the registers the staging has just
+// freed; past the launch's end the last tile is loaded again, so no branch
+// makes the compiler wait for the loads early).  FP32 rows of n = 2 HSP
+// points in whole 16-row tiles and the FP64 factor only (the host checks).
+template <typename Tin, int HSP>
+__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2p(const DenseOp *__restrict__ ops, int ntiles, int tpo)
+{
+    constexpr int LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
+    __shared__ double Ms[HSP * LDM];
+    __shared__ double Ma[HSP * LDM];
+    __shared__ __attribute__((aligned(16))) double X[kMfma2Waves * 8 * XS];
+    mfma2_stage_ops<HSP>(ops[0], Ms, Ma, threadIdx.x, 64 * kMfma2Waves);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double *Xw = X + wave * 8 * XS;
+    int t = blockIdx.x * kMfma2Waves + wave;
+    if (t >= ntiles) return;   // the whole wave; no barrier follows
+    // only the slice (read once, from HBM) is prefetched; the factor, reused
+    // by every step of a block, is loaded at staging
+    Pair<Tin> xn[16];
+    auto load_x = [&](int u) {
+        const DenseOp &o = ops[u / tpo];
+        const int rb = (u % tpo) * 16, n = o.n, nr = min(16, o.nrows - rb);
+        const unsigned o0 = unsigned(rb) * unsigned(n) + unsigned(2 * lane < n ? 2 * lane : 0);
+        const Tin *src = static_cast<const Tin *>(o.src);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) xn[r] = ldg_pair_nt(src, o0 + unsigned(r < nr ? r : 0) * unsigned(n));
+    };
+    load_x(t);
+    for (; t < ntiles; t += gridDim.x * kMfma2Waves) {
+        const DenseOp &op = ops[t / tpo];
+        const int rbase = (t % tpo) * 16;
+        Z2Regs<Tin> R;
+        {
+            const int n = op.n, nr = min(16, op.nrows - rbase);
+            const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(2 * lane < n ? 2 * lane : 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                R.xa[r] = xn[r];
+                R.ea[r] = ldg_pair(op.epre, o0 + unsigned(r < nr ? r : 0) * unsigned(n));
+            }
+        }
+        double sf[HSP / 4], af[HSP / 4];
+        mfma2_stage<Tin, HSP>(op, rbase, R, Xw, lane, sf, af);
+        load_x(min(t + int(gridDim.x) * kMfma2Waves, ntiles - 1));
+        mfma2_mma_store<float, HSP, true>(op, rbase, static_cast<float *>(op.dst), Ms, Ma, Xw, lane, sf, af);
     }
 }
 

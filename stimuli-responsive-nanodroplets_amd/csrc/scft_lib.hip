@@ -230,6 +230,7 @@ struct Launch {
     size_t lds;
     FilmTab ftab{};  // tables of a fused film step (LK_FILM)
     int step = 0;    // contour step
+    bool pf = false; // LK_DENSE_MFMA2: the kernel form with the next tile's slice rows in flight
 };
 
 bool fftwave_supported(int L)
@@ -254,6 +255,7 @@ struct scft_ctx {
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> evs;
     bool par_axes = true;            // SCFT_PARX=0: one after another
+    bool zpf = true;                 // z operator with the next tile's slice rows in flight (SCFT_ZPF=0: off)
     std::vector<Slot> slots;
     std::vector<void *> allocs;
     std::vector<std::unique_ptr<FftTable>> ffts;
@@ -936,6 +938,11 @@ int build_schedule(scft_ctx *c)
                     // persistent waves over (op, 16-row tile); one workgroup per CU
                     const bool even = td[0].hs == td[0].ha && !c->dense_mfma1;
                     const int wpg = even ? kMfma2Waves : kMfmaWaves;
+                    // the prefetching form takes FP32 rows of n = 2 HSP in whole 16-row tiles
+                    // and the FP64 factor
+                    bool pf = even && c->zpf && ts == 4 && !c->tmp_wide && (hsp == 64 || hsp == 32);
+                    for (auto &op : td) pf &= op.n == 2 * hsp && op.nrows % 16 == 0 && !op.e2f;
+                    ln.pf = pf;
                     ln.kind = even ? LK_DENSE_MFMA2 : LK_DENSE_MFMA;
                     ln.hsp = hsp;
                     ln.P = (maxl + 15) / 16;   // tiles per op
@@ -1173,6 +1180,13 @@ void launch_one(scft_ctx *c, const Launch &ln, hipStream_t st)
         const DenseOp *ops = c->denseops + ln.first;
         const dim3 g1(ln.gx), b(64 * kMfma2Waves);
         const int nt = ln.count * ln.P;
+        if constexpr (sizeof(Tout) == 4) {
+            if (ln.pf) {
+                if (ln.hsp == 64) hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 64>), g1, b, 0, st, ops, nt, ln.P);
+                else hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32>), g1, b, 0, st, ops, nt, ln.P);
+                break;
+            }
+        }
         if (ln.hsp == 16) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 16>), g1, b, 0, st, ops, nt, ln.P);
         else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 32>), g1, b, 0, st, ops, nt, ln.P);
         else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 48>), g1, b, 0, st, ops, nt, ln.P);
@@ -1705,6 +1719,10 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     }
     const char *px = std::getenv("SCFT_PARX");
     c->par_axes = !(px && px[0] == '0');
+    // SCFT_ZPF=0: the MFMA z operator without the next tile's slice rows in
+    // flight (k_zdense_mfma2 instead of k_zdense_mfma2p)
+    const char *zp = std::getenv("SCFT_ZPF");
+    c->zpf = !(zp && zp[0] == '0');
     // the FFT tiles may use more than 64 KB of LDS on gfx950 (160 KB per workgroup)
     const void *lk[] = {(const void *)k_lineop<double, double>, (const void *)k_lineop<float, double>,
                         (const void *)k_lineop<double, float>, (const void *)k_lineop<float, float>};
@@ -2292,15 +2310,17 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
 int scft_get_schedule(const scft_ctx *c, char *buf, int len)
 {
     if (!c || !buf || len <= 0) return SCFT_EINVAL;
-    int n[kLaunchKinds] = {};
-    for (auto &ln : c->launches)
+    int n[kLaunchKinds] = {}, npf = 0;
+    for (auto &ln : c->launches) {
         if (ln.kind >= 0 && ln.kind < kLaunchKinds) n[ln.kind]++;
+        npf += ln.kind == LK_DENSE_MFMA2 && ln.pf;
+    }
     std::snprintf(buf, len,
                   "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fftreg=%d dense_mfma=%d dense_mfma2=%d graph=%d "
-                  "fftreg2=%d plane=%d film=%d fftx=%d",
+                  "fftreg2=%d plane=%d film=%d fftx=%d mfma2_pf=%d",
                   c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFTREG],
                   n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0, n[LK_FFTREG2], n[LK_PLANE],
-                  n[LK_FILM], n[LK_FFTX]);
+                  n[LK_FILM], n[LK_FFTX], npf);
     return SCFT_OK;
 }
 
