@@ -55,7 +55,7 @@ def measured_traffic(schedule, slots, precision):
         return None
     with open(TRAFFIC_PROFILE) as f:
         t = json.load(f)
-    if t.get("kernel") != "k_zdense_mfma2" or "dense_mfma2=100" not in schedule:
+    if t.get("kernel") != dominant_kernel(schedule).split()[0] or "dense_mfma2=100" not in schedule:
         return None
     return float(t["bytes_per_launch"])
 

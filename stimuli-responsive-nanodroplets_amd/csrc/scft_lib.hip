@@ -1499,11 +1499,11 @@ int autotune_z(scft_ctx *c)
         auto it = g_tune.find(key);
         if (it != g_tune.end()) { apply(it->second); return SCFT_OK; }
     }
-    // time of the launches of contour step 0 and of the middle step with `ns`
-    // slots active; SCFT_TUNE_LOG=1: every timing to stderr
+    // time per contour step with `ns` slots active (below); SCFT_TUNE_LOG=1:
+    // every timing to stderr
     const char *tl = std::getenv("SCFT_TUNE_LOG");
     const bool tune_log = tl && tl[0] == '1';
-    auto time_step0 = [&](int kind, int ns, double *ms_out, bool xcoal = false) -> int {
+    auto time_steps = [&](int kind, int ns, double *ms_out, bool xcoal = false) -> int {
         c->zkind = kind == 0 ? Z_FFT : Z_DENSE;
         for (int i = 0; i < 3; ++i) c->film_cls[i] = kind == 2;
         for (int i = 0; i < 3; ++i) c->xcoal_cls[i] = xcoal;
@@ -1516,52 +1516,88 @@ int autotune_z(scft_ctx *c)
             }
         }
         RETCHK(build_schedule(c));
-        // the launches of contour step 0 (every chain) and of the middle step
-        // (the chains that run the whole contour)
-        std::vector<Launch> zl(c->launches.begin(), c->launches.begin() + c->nstep0);
+        // Contour steps come in two kinds: those where every chain is live
+        // (the first n0) and, when some chains are shorter, the rest.  Each
+        // kind is timed on a window of up to 20 consecutive steps (the first
+        // ones, and those around the middle step) and weighted by its number
+        // of steps, so that a form that is slow only on the few steps of
+        // short chains is not charged for every step.  Consecutive steps,
+        // each reading the slice the one before wrote, as the iteration runs
+        // them: one step repeated timed the fused film step 46% slow.
         const int nst = int(c->step_first.size());
-        if (nst > 2) {
-            const int a = c->step_first[nst / 2], b = c->step_first[nst / 2 + 1];
-            zl.insert(zl.end(), c->launches.begin() + a, c->launches.begin() + b);
+        if (nst == 0) {
+            *ms_out = 0.0;
+            return SCFT_OK;
         }
-        std::swap(zl, c->launches);
-        // 20 repetitions of those launches as one graph, as the iteration
-        // runs them: eager launches of small kernels would time the host
+        auto live = [&](int t) {
+            int n = 0;
+            for (int si : c->act)
+                for (const Chain &ch : c->slots[si].chains) n += ch.nsteps > t;
+            return n;
+        };
+        const int l0 = live(0);
+        int n0 = 0;
+        for (int t = 0; t < nst; ++t) n0 += live(t) == l0;
+        const int tm = nst / 2;
+        const bool two = n0 < nst && live(tm) != l0;
+        // the launches of steps [t0, t1)
+        auto step_launches = [&](int t0, int t1) {
+            const int a = c->step_first[t0];
+            const int b = t1 < nst ? c->step_first[t1] : int(c->launches.size());
+            return std::vector<Launch>(c->launches.begin() + a, c->launches.begin() + b);
+        };
+        // the window's launches as one graph, as the iteration runs them
+        // (eager launches of small kernels would time the host); the fastest
+        // of three replays, in us per step
         const bool f32 = c->tsize == 4;
-        hipGraph_t g = nullptr;
-        hipGraphExec_t gx = nullptr;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            for (int r = 0; r < 20; ++r) f32 ? launch_steps<float>(c, true) : launch_steps<double>(c, true);
-            e = hipStreamEndCapture(c->stream, &g);
+        auto time_graph = [&](int t0, int t1, double *us) -> int {
+            std::vector<Launch> zl = step_launches(t0, t1);
+            std::swap(zl, c->launches);
+            hipGraph_t g = nullptr;
+            hipGraphExec_t gx = nullptr;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                f32 ? launch_steps<float>(c, true) : launch_steps<double>(c, true);
+                e = hipStreamEndCapture(c->stream, &g);
+            }
+            std::swap(zl, c->launches);
+            if (e == hipSuccess) e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+            if (e == hipSuccess) e = hipEventCreate(&e0);
+            if (e == hipSuccess) e = hipEventCreate(&e1);
+            if (e == hipSuccess) e = hipGraphLaunch(gx, c->stream);   // warm
+            float ms = 1e30f;
+            for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+                float m = 0.f;
+                e = hipEventRecord(e0, c->stream);
+                if (e == hipSuccess) e = hipGraphLaunch(gx, c->stream);
+                if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+                ms = std::min(ms, m);
+            }
+            if (e1) (void)hipEventDestroy(e1);
+            if (e0) (void)hipEventDestroy(e0);
+            if (gx) (void)hipGraphExecDestroy(gx);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                set_err(std::string("autotune timing: ") + hipGetErrorString(e));
+                return SCFT_EHIP;
+            }
+            *us = 1000.0 * ms / (t1 - t0);
+            return SCFT_OK;
+        };
+        double u0 = 0.0, um = 0.0;
+        RETCHK(time_graph(0, std::min(n0, 20), &u0));
+        if (two) {
+            const int a = std::max(n0, tm - 10), b = std::min(nst, a + 20);
+            RETCHK(time_graph(a, b, &um));
         }
-        std::swap(zl, c->launches);
-        if (e == hipSuccess) e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
-        if (e == hipSuccess) e = hipEventCreate(&e0);
-        if (e == hipSuccess) e = hipEventCreate(&e1);
-        if (e == hipSuccess) e = hipGraphLaunch(gx, c->stream);   // warm
-        float ms = 1e30f;
-        for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {   // the fastest of three
-            float m = 0.f;
-            e = hipEventRecord(e0, c->stream);
-            if (e == hipSuccess) e = hipGraphLaunch(gx, c->stream);
-            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
-            ms = std::min(ms, m);
-        }
-        if (e1) (void)hipEventDestroy(e1);
-        if (e0) (void)hipEventDestroy(e0);
-        if (gx) (void)hipGraphExecDestroy(gx);
-        if (g) (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            set_err(std::string("autotune timing: ") + hipGetErrorString(e));
-            return SCFT_EHIP;
-        }
+        const double ms = two ? (n0 * u0 + (nst - n0) * um) / nst : u0;
         if (tune_log)
-            std::fprintf(stderr, "scft autotune: NX %d NY %d NZ %d, %d slots, z form %d, k_fftx %d: %.2f us per first "
-                         "and middle step\n", d0.nx, c->NY, c->NZ, ns, kind, xcoal ? 1 : 0, 1000.0 * ms / 20.0);
+            std::fprintf(stderr, "scft autotune: NX %d NY %d NZ %d, %d slots, z form %d, k_fftx %d: %.2f us per step "
+                         "(%d steps of %.2f, %d of %.2f)\n", d0.nx, c->NY, c->NZ, ns, kind, xcoal ? 1 : 0, ms, n0, u0,
+                         two ? nst - n0 : 0, um);
         *ms_out = ms;
         return SCFT_OK;
     };
@@ -1573,7 +1609,7 @@ int autotune_z(scft_ctx *c)
     for (int kind : cands) {
         if (cands.size() == 1) break;
         double ms;
-        RETCHK(time_step0(kind, nt, &ms));
+        RETCHK(time_steps(kind, nt, &ms));
         if (kind == 2) ms /= kFilmMargin;
         if (ms < best) { best = ms; choice = kind; }
     }
@@ -1588,8 +1624,8 @@ int autotune_z(scft_ctx *c)
         bool f = choice == 2;
         if (ns < nt && has_film && has_split) {
             double t1, t2;
-            RETCHK(time_step0(1, ns, &t1));
-            RETCHK(time_step0(2, ns, &t2));
+            RETCHK(time_steps(1, ns, &t1));
+            RETCHK(time_steps(2, ns, &t2));
             f = t2 < kFilmMargin * t1;
         }
         if (f) result |= 1 << (2 + k);
@@ -1606,8 +1642,8 @@ int autotune_z(scft_ctx *c)
             if (it == at.end()) {
                 const int kind = choice == 0 ? 0 : 1;
                 double t1, t2;
-                RETCHK(time_step0(kind, ns, &t1, false));
-                RETCHK(time_step0(kind, ns, &t2, true));
+                RETCHK(time_steps(kind, ns, &t1, false));
+                RETCHK(time_steps(kind, ns, &t2, true));
                 it = at.emplace(ns, t2 < kFilmMargin * t1).first;
             }
             if (it->second) result |= 1 << (5 + k);
