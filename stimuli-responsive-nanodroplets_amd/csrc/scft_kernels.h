@@ -696,9 +696,10 @@ __device__ __forceinline__ void mfma2_stage(const DenseOp &op, int rbase, const 
 // The products of a staged tile and its stores to dst (the op's output,
 // or a caller's scratch with the same layout).  Ms/Ma are the staged
 // operator halves, Xw the wave's staging rows (reused for the FP32 rows).
-// ROWS: the caller guarantees n = 2 HSP and whole 16-row tiles (FP32
-// output), so the row-store path is the only one compiled.
-template <typename Tout, int HSP, bool ROWS = false>
+// NROW > 0: the caller guarantees n = NROW (a multiple of 16, at most
+// 2 HSP) and whole 16-row tiles with FP32 output, so the row-store path is
+// the only one compiled, with a fixed number of stores.
+template <typename Tout, int HSP, int NROW = 0>
 __device__ __forceinline__ void mfma2_mma_store(const DenseOp &op, int rbase, Tout *dst, const double *Ms,
                                                 const double *Ma, double *Xw, int lane, const double (&sf)[HSP / 4],
                                                 const double (&af)[HSP / 4])
@@ -725,26 +726,28 @@ __device__ __forceinline__ void mfma2_mma_store(const DenseOp &op, int rbase, To
     }
     // D layout: register r of lane l is row l/16 + 4r, output point l%16
     // of the column tile; the mirrored half goes to n-1-k
-    static_assert(!ROWS || (sizeof(Tout) == 4 && (HSP == 64 || HSP == 32)), "row stores: FP32 rows of 64 or 128");
-    if constexpr (sizeof(Tout) == 4 && (HSP == 64 || HSP == 32)) {
-        if (ROWS || (n == 2 * HSP && nr == 16)) {
-            // FP32 rows of n = 2 HSP points: the tile's results go through the
-            // wave's staging rows (16 rows of n floats, the 16-byte chunks of
-            // row rr permuted by 4 (rr & 3), so that the four rows one write
-            // phase touches fall into different banks) and out as whole rows,
-            // 16 bytes per lane, instead of 4-byte stores of 64-byte runs
-            constexpr int RC = 2 * HSP / 4;            // 16-byte chunks per row
-            constexpr int RPI = 64 / RC;               // rows per store instruction
+    static_assert(NROW == 0 || (sizeof(Tout) == 4 && NROW % 16 == 0 && NROW <= 2 * HSP), "row stores: FP32 rows");
+    if constexpr (sizeof(Tout) == 4 && (NROW > 0 || HSP == 64 || HSP == 32)) {
+        if (NROW > 0 || (n == 2 * HSP && nr == 16)) {
+            // FP32 rows of NR points: the tile's results go through the wave's
+            // staging rows (16 rows at a pitch of 2 HSP floats, the 16-byte
+            // chunks of row rr permuted by 4 (rr & 3), so that the four rows
+            // one write phase touches fall into different banks) and out as
+            // whole rows, 16 bytes per lane, instead of 4-byte stores of
+            // 64-byte runs
+            constexpr int NR = NROW > 0 ? NROW : 2 * HSP;
+            constexpr int RC = NR / 4;   // 16-byte chunks per row
             float *Yw = reinterpret_cast<float *>(Xw);
             wave_sync();   // every lane has read its fragments from the staging rows
 #pragma unroll
             for (int kt = 0; kt < NK; ++kt) {
                 const int k = 16 * kt + li;
+                if (NR < 2 * HSP && k >= NR / 2) continue;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int rr = lj + 4 * r, f = 4 * (rr & 3);
                     const double u = U[kt][r], v = W[kt][r];
-                    const int k2 = 2 * HSP - 1 - k;
+                    const int k2 = NR - 1 - k;
                     Yw[rr * 2 * HSP + ((((k >> 2) ^ f)) << 2) + (k & 3)] = Tout(u + v);
                     Yw[rr * 2 * HSP + ((((k2 >> 2) ^ f)) << 2) + (k2 & 3)] = Tout(u - v);
                 }
@@ -752,11 +755,11 @@ __device__ __forceinline__ void mfma2_mma_store(const DenseOp &op, int rbase, To
             wave_sync();
             typedef float f4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-            for (int it = 0; it < 16 / RPI; ++it) {
-                const int rr = RPI * it + lane / RC, c = lane % RC;
+            for (int it = 0; it < NR / 16; ++it) {
+                const int idx = 64 * it + lane, rr = idx / RC, c = idx % RC;
                 const f4 x = *reinterpret_cast<const f4 *>(Yw + rr * 2 * HSP + ((c ^ (4 * (rr & 3))) << 2));
                 *(__attribute__((address_space(1))) f4 *)((char *)dst +
-                                                          (unsigned(rbase + rr) * unsigned(n) + unsigned(4 * c)) * 4u) = x;
+                                                          (unsigned(rbase + rr) * unsigned(NR) + unsigned(4 * c)) * 4u) = x;
             }
             return;
         }
@@ -811,9 +814,10 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
 // The same with the next tile's loads in flight under each tile's products
 // (issued once the tile is staged, into the registers the staging has just
 // freed; past the launch's end the last tile is loaded again, so no branch
-// makes the compiler wait for the loads early).  FP32 rows of n = 2 HSP
-// points in whole 16-row tiles and the FP64 factor only (the host checks).
-template <typename Tin, int HSP>
+// makes the compiler wait for the loads early).  FP32 rows of n = N points
+// (N a multiple of 16, at most 2 HSP) in whole 16-row tiles and the FP64
+// factor only (the host checks).
+template <typename Tin, int HSP, int N>
 __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2p(const DenseOp *__restrict__ ops, int ntiles, int tpo)
 {
     constexpr int LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
@@ -855,7 +859,7 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2p(const DenseO
         double sf[HSP / 4], af[HSP / 4];
         mfma2_stage<Tin, HSP>(op, rbase, R, Xw, lane, sf, af);
         load_x(min(t + int(gridDim.x) * kMfma2Waves, ntiles - 1));
-        mfma2_mma_store<float, HSP, true>(op, rbase, static_cast<float *>(op.dst), Ms, Ma, Xw, lane, sf, af);
+        mfma2_mma_store<float, HSP, N>(op, rbase, static_cast<float *>(op.dst), Ms, Ma, Xw, lane, sf, af);
     }
 }
 

@@ -938,11 +938,14 @@ int build_schedule(scft_ctx *c)
                     // persistent waves over (op, 16-row tile); one workgroup per CU
                     const bool even = td[0].hs == td[0].ha && !c->dense_mfma1;
                     const int wpg = even ? kMfma2Waves : kMfmaWaves;
-                    // the prefetching form takes FP32 rows of n = 2 HSP in whole 16-row tiles
-                    // and the FP64 factor
-                    bool pf = even && c->zpf && ts == 4 && !c->tmp_wide && (hsp == 64 || hsp == 32);
-                    for (auto &op : td) pf &= op.n == 2 * hsp && op.nrows % 16 == 0 && !op.e2f;
+                    // the prefetching form takes FP32 rows of NZ 128, 64 or 48 in whole 16-row
+                    // tiles and the FP64 factor
+                    const int n0 = td[0].n;
+                    bool pf = even && c->zpf && ts == 4 && !c->tmp_wide &&
+                              ((hsp == 64 && n0 == 128) || (hsp == 32 && (n0 == 64 || n0 == 48)));
+                    for (auto &op : td) pf &= op.n == n0 && op.nrows % 16 == 0 && !op.e2f;
                     ln.pf = pf;
+                    if (pf) ln.L = n0;   // the row length of the prefetching form
                     ln.kind = even ? LK_DENSE_MFMA2 : LK_DENSE_MFMA;
                     ln.hsp = hsp;
                     ln.P = (maxl + 15) / 16;   // tiles per op
@@ -1182,8 +1185,9 @@ void launch_one(scft_ctx *c, const Launch &ln, hipStream_t st)
         const int nt = ln.count * ln.P;
         if constexpr (sizeof(Tout) == 4) {
             if (ln.pf) {
-                if (ln.hsp == 64) hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 64>), g1, b, 0, st, ops, nt, ln.P);
-                else hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32>), g1, b, 0, st, ops, nt, ln.P);
+                if (ln.hsp == 64) hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 64, 128>), g1, b, 0, st, ops, nt, ln.P);
+                else if (ln.L == 64) hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32, 64>), g1, b, 0, st, ops, nt, ln.P);
+                else hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32, 48>), g1, b, 0, st, ops, nt, ln.P);
                 break;
             }
         }

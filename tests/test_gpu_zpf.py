@@ -1,10 +1,11 @@
 """The MFMA z operator with the next tile's slice rows in flight
-(k_zdense_mfma2p, the default for FP32 slices of NZ 64 and 128) against
+(k_zdense_mfma2p, the default for FP32 slices of NZ 48, 64 and 128) against
 the form that loads each tile when it starts (SCFT_ZPF=0): the same
 arithmetic, so results are bit-identical.
 
-- C2 films (NZ 128) and a 3D grid of NZ 64 with uneven NX, FP32-split,
-  three iterations: F of every slot and every slot's fields;
+- C2 films (NZ 128) and 3D grids of NZ 64 and 48 (rows shorter than the
+  operator's padded width) with uneven NX, FP32-split, three iterations:
+  F of every slot and every slot's fields;
 - FP64 slices keep the plain form (the schedule reports no prefetching
   launches).
 The prefetching form also runs in every FP32-split parity test against the
@@ -55,13 +56,15 @@ CASES = {
     "c2": lambda configs: [configs.params_c2(chi_ab=25.0 + s) for s in range(3)],
     "nz64": lambda configs: [configs.chem_droplet(lx=lx, ly=4.0, lz=16.0, dr=0.25, ns=21, vol_dbk=0.2, vol_s1=0.05,
                                                   len_s1=0.3) for lx in (4.0, 6.0, 4.0)],
+    "nz48": lambda configs: [configs.chem_droplet(lx=lx, ly=4.0, lz=12.0, dr=0.25, ns=21, vol_dbk=0.2, vol_s1=0.05,
+                                                  len_s1=0.3) for lx in (4.0, 6.0, 4.0)],
 }
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_prefetching_z_operator_is_bit_identical(abi, configs, lib, case):
     protos = CASES[case](configs)
-    assert abi.grid_of(protos[0])[2] in (64, 128)
+    assert abi.grid_of(protos[0])[2] in (48, 64, 128)
     sa, Fa, wa = _run(abi, configs, lib, protos, abi.SCFT_FP32_SPLIT, True)
     sb, Fb, wb = _run(abi, configs, lib, protos, abi.SCFT_FP32_SPLIT, False)
     print(sa, "|", sb, flush=True)
