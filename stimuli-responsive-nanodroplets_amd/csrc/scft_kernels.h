@@ -1735,10 +1735,99 @@ __device__ __forceinline__ int film_at(int e, int c)
 }
 
 // The z operator on rows rbase..rbase+15 for this part's 16 symmetric
-// output columns (mfma2_tile's loads, staging and fragments; bs/ba hold the
-// part's column of Ms^T / Ma^T for each K step); results rounded to T into
-// the part's LDS rows.  The slice is loaded as ordinary (cached) data: the
-// other parts of the plane read the same rows.
+// output columns (mfma2_tile's staging and fragments; bs/ba hold the part's
+// column of Ms^T / Ma^T for each K step); results rounded to T into the
+// part's LDS rows.  The slice is loaded as ordinary (cached) data: the
+// other parts of the plane read the same rows.  film_zload loads a tile's
+// slice rows (the caller keeps the next tile's in flight), film_zstage
+// applies the step's factor (loaded there: every step of a block reuses
+// it) and builds the fragments, film_zmma runs the products.
+template <typename T>
+__device__ __forceinline__ void film_zload(const FilmOp &op, int rbase, int lane, Pair<T> (&xa)[16])
+{
+    const T *src = static_cast<const T *>(op.src);
+    const unsigned o0 = unsigned(rbase) * unsigned(kFilmN) + unsigned(2 * lane);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) xa[r] = ldg_pair(src, o0 + unsigned(r) * unsigned(kFilmN));
+}
+
+template <typename T>
+__device__ __forceinline__ void film_zstage(const FilmOp &op, int rbase, const Pair<T> (&xa)[16], double *Xw,
+                                            int lane, double (&sf)[16], double (&af)[16])
+{
+    constexpr int n = kFilmN, NT = 16, XS = Mfma2<64>::XS;
+    const int li = lane & 15, lj = lane >> 4;
+    const int c = 2 * lane;   // this lane's point pair in a row
+    const int mode = !op.epre2 ? 1 : (op.epre2 == op.epre ? (op.e2f ? 4 : 2) : 3);
+    const unsigned o0 = unsigned(rbase) * unsigned(n) + unsigned(c);
+    Pair<double> ea[16];
+    if (mode == 4) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const Pair<float> f = ldg_pair(op.e2f, o0 + unsigned(r) * unsigned(n));
+            ea[r] = Pair<double>{double(f.x), double(f.y)};
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) ea[r] = ldg_pair(op.epre, o0 + unsigned(r) * unsigned(n));
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        wave_sync();   // the previous reads of the staging rows are done
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int R = 8 * h + r;
+            double e0 = ea[R].x, e1 = ea[R].y;
+            if (mode == 2) {
+                e0 *= e0;
+                e1 *= e1;
+            } else if (mode == 3) {   // one step per block: the second factor
+                const Pair<double> f = ldg_pair(op.epre2, o0 + unsigned(R) * unsigned(n));
+                e0 *= f.x;
+                e1 *= f.y;
+            }
+            double2 v;
+            v.x = double(xa[R].x) * e0;
+            v.y = double(xa[R].y) * e1;
+            *reinterpret_cast<double2 *>(&Xw[r * XS + c]) = v;
+        }
+        wave_sync();
+        if ((li >> 3) == h) {
+            const double *row = Xw + (li & 7) * XS;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                const int j = lj * NT + k;
+                const double x1 = row[j], x2 = row[n - 1 - j];
+                sf[k] = x1 + x2;
+                af[k] = x1 - x2;
+            }
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void film_zmma(int rbase, const double (&bs)[16], const double (&ba)[16],
+                                          const double (&sf)[16], const double (&af)[16], T *Tm, int lane)
+{
+    const int li = lane & 15, lj = lane >> 4;
+    d4_t U = d4_t{0.0, 0.0, 0.0, 0.0}, W = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        U = __builtin_amdgcn_mfma_f64_16x16x4f64(sf[k], bs[k], U, 0, 0, 0);
+        W = __builtin_amdgcn_mfma_f64_16x16x4f64(af[k], ba[k], W, 0, 0, 0);
+    }
+    // register r of lane l: row l/16 + 4r; output column li of the part
+    // (z = k0 + li) and its mirror (z = n-1-k0-li, local column 31 - li)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = rbase + lj + 4 * r;
+        Tm[film_at<T>(e, li)] = T(U[r] + W[r]);
+        Tm[film_at<T>(e, 31 - li)] = T(U[r] - W[r]);
+    }
+}
+
+// The same in one piece (loads, staging, products), for the forms without
+// the prefetch.
 template <typename T>
 __device__ __forceinline__ void film_ztile(const FilmOp &op, int rbase, const double (&bs)[16], const double (&ba)[16],
                                            double *Xw, T *Tm, int lane)
@@ -2021,7 +2110,8 @@ __global__ void __launch_bounds__(64 * kFilmWaves) k_fftx(const LineOp *__restri
 // grid: 32 ceil(nops / 8) workgroups; block b runs part (b / 8) % 4 of op
 // 8 (b / 32) + b % 8
 template <int L, typename T>
-__global__ void __launch_bounds__(64 * kFilmWaves) k_film(const FilmOp *__restrict__ ops, int nops, FilmTab tab)
+__global__ void __launch_bounds__(64 * kFilmWaves, (sizeof(T) == 4 && L <= 256) ? 2 : 1)
+    k_film(const FilmOp *__restrict__ ops, int nops, FilmTab tab)
 {
     constexpr int NTILE = L / 16, XS = Mfma2<64>::XS;
     __shared__ __attribute__((aligned(16))) T Tm[L * kFilmCols];
@@ -2045,7 +2135,23 @@ __global__ void __launch_bounds__(64 * kFilmWaves) k_film(const FilmOp *__restri
         ba[k] = ldg(tab.MaT, a);
     }
     double *Xw = X + wave * 8 * XS;
-    for (int t = wave; t < NTILE; t += kFilmWaves) film_ztile<T>(op, 16 * t, bs, ba, Xw, Tm, lane);
+    // each wave takes NTILE / 4 tiles.  FP32 slices of x length up to 256
+    // (two workgroups per CU in 256 registers): the next tile's slice rows
+    // load while this tile's products run (past the last tile the last is
+    // loaded again); otherwise each tile loads when it starts (the
+    // prefetch there would cost a wave per SIMD or spill)
+    if constexpr (sizeof(T) == 4 && L <= 256) {
+        Pair<T> xa[16];
+        film_zload<T>(op, 16 * wave, lane, xa);
+        for (int t = wave; t < NTILE; t += kFilmWaves) {
+            double sf[16], af[16];
+            film_zstage<T>(op, 16 * t, xa, Xw, lane, sf, af);
+            film_zload<T>(op, 16 * min(t + kFilmWaves, NTILE - 1), lane, xa);
+            film_zmma<T>(16 * t, bs, ba, sf, af, Tm, lane);
+        }
+    } else {
+        for (int t = wave; t < NTILE; t += kFilmWaves) film_ztile<T>(op, 16 * t, bs, ba, Xw, Tm, lane);
+    }
     __syncthreads();   // every row of the part's columns is in LDS
     const int g = lane & 3, p = lane >> 2;
     const int i = 4 * wave + g;   // complex line: local columns 2i, 2i + 1
