@@ -504,8 +504,9 @@ class Server:
             # library reads them when a batch is created):
             # - SCFT_FILM=0: the fused film step wins on an idle GPU where few
             #   chains share a launch, but next to another pool's kernels it
-            #   lost: at 4 requests/s of the 80/20 2D/3D stream, 2D p50 1.64 s
-            #   with it against 1.28 s with the z and x launches apart;
+            #   lost: at 4 and 6 requests/s of the 80/20 2D/3D stream, 2D p50
+            #   1.58 and 2.68 s with it against 1.28 and 2.20 s with the z and
+            #   x launches apart;
             # - SCFT_PARX=0: a pool of uneven NX would run its x launches side
             #   by side on extra streams, which share the process's few
             #   hardware queues with the other pools' streams: p50 3.37 /
