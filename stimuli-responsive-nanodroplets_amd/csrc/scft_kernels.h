@@ -2573,32 +2573,39 @@ struct UpdArgs {
     double *partial;     // [2][nblocks]: sum p_new, sum dw^2
 };
 
-// funUpdate per voxel (SCFT.c:2047-2113); grid: (nblocks, instances)
-__global__ void __launch_bounds__(kThreads) k_update(const UpdArgs *__restrict__ args)
+// funUpdate for the voxels of one thread (SCFT.c:2047-2113), with NA the
+// number of active fields when it is known at compile time (the loops then
+// unroll and the per-voxel arrays stay in registers) or 0 (a.nact)
+template <int NA>
+__device__ __forceinline__ void update_voxels(const UpdArgs &a, double (&acc)[2])
 {
-    const UpdArgs &a = args[blockIdx.y];
     const int V = a.V;
-    double acc[2] = {0.0, 0.0};
+    const int n = NA > 0 ? NA : a.nact;
     for (int idx = blockIdx.x * kThreads + threadIdx.x; idx < V; idx += gridDim.x * kThreads) {
-        double w0[13], r[13], dW[13];
-        const int n = a.nact;
+        double w0[NA > 0 ? NA : 13], r[NA > 0 ? NA : 13], dW[NA > 0 ? NA : 13];
+#pragma unroll
         for (int i = 0; i < n; ++i) {
             const int f = a.act[i];
             w0[i] = a.w[(size_t)f * V + idx];
             r[i] = a.tot[(size_t)f * V + idx] - a.phibar[f];
         }
         const double pv = a.p[idx];
+#pragma unroll
         for (int i = 0; i < n; ++i) {
             double d = -w0[i] + pv;
+#pragma unroll
             for (int j = 0; j < n; ++j)
                 if (j != i) d += a.chi[a.act[i]][a.act[j]] * r[j];
             dW[i] = d;
         }
         double corr = 0.0;
         if (n > 1)
+#pragma unroll
             for (int i = 0; i < n; ++i)
+#pragma unroll
                 for (int j = i + 1; j < n; ++j) {
                     const double c = a.chi[a.act[i]][a.act[j]];
+#pragma unroll
                     for (int h = 0; h < n; ++h)
                         if (h != i && h != j) corr += c * r[h];
                 }
@@ -2608,8 +2615,10 @@ __global__ void __launch_bounds__(kThreads) k_update(const UpdArgs *__restrict__
             draw = ((size_t)(k * a.ny + j) * a.nx + i) * n;
         }
         double psum = 0.0;
+#pragma unroll
         for (int i = 0; i < n; ++i) {
             double wn = w0[i];
+#pragma unroll
             for (int j = 0; j < n; ++j) wn += ((i == j) ? a.g1 : a.g2) * dW[j];
             if (a.noise) wn += a.amp * a.noise[draw + i];
             a.w[(size_t)a.act[i] * V + idx] = wn;
@@ -2619,6 +2628,21 @@ __global__ void __launch_bounds__(kThreads) k_update(const UpdArgs *__restrict__
         const double pn = (psum + corr) / n;
         a.p[idx] = pn;
         acc[0] += pn;
+    }
+}
+
+// funUpdate per voxel (SCFT.c:2047-2113); grid: (nblocks, instances)
+__global__ void __launch_bounds__(kThreads) k_update(const UpdArgs *__restrict__ args)
+{
+    const UpdArgs &a = args[blockIdx.y];
+    double acc[2] = {0.0, 0.0};
+    switch (a.nact) {   // uniform per workgroup
+    case 2: update_voxels<2>(a, acc); break;
+    case 3: update_voxels<3>(a, acc); break;
+    case 4: update_voxels<4>(a, acc); break;
+    case 5: update_voxels<5>(a, acc); break;
+    case 6: update_voxels<6>(a, acc); break;
+    default: update_voxels<0>(a, acc); break;
     }
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
