@@ -36,7 +36,7 @@ namespace scft {
 
 constexpr int kThreads = 256;
 constexpr int kMaxTerms = 24;   // density terms (species blocks)
-constexpr int kRedBlocks = 256; // partial sums per reduction and instance
+constexpr int kRedBlocks = 1024; // partial sums (workgroups) per reduction and instance
 constexpr int kDenseTM = 64;    // rows per k_zdense workgroup
 constexpr int kDenseHMax = 64;  // k_zdense handles NZ <= 128
 
