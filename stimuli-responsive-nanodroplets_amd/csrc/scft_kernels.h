@@ -2664,14 +2664,27 @@ struct FinArgs {
     Scalars *out;
 };
 
-// Q from its partials; grid: (instances), block (kThreads): thread s < nq sums slot s.
+// Q from its partials; grid: (instances), block (kThreads).  Up to 256
+// partials: thread s < nq sums slot s in order.  More (grids of more than
+// 64 K voxels): wave w sums slots w, w + 4, ... across its lanes, so that
+// no thread adds 1024 partials one after another.
 __global__ void __launch_bounds__(kThreads) k_qfinal(const FinArgs *__restrict__ args)
 {
     const FinArgs &a = args[blockIdx.x];
-    for (int s = threadIdx.x; s < a.nq; s += kThreads) {
+    if (a.nblocks <= 256) {
+        for (int s = threadIdx.x; s < a.nq; s += kThreads) {
+            double v = 0.0;
+            for (int b = 0; b < a.nblocks; ++b) v += a.qpart[s * a.nblocks + b];
+            a.out->Q[a.qsp[s]] = v / a.V;
+        }
+        return;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int s = wave; s < a.nq; s += kThreads / 64) {
         double v = 0.0;
-        for (int b = 0; b < a.nblocks; ++b) v += a.qpart[s * a.nblocks + b];
-        a.out->Q[a.qsp[s]] = v / a.V;
+        for (int b = lane; b < a.nblocks; b += 64) v += a.qpart[s * a.nblocks + b];
+        v = wave_sum(v);
+        if (lane == 0) a.out->Q[a.qsp[s]] = v / a.V;
     }
 }
 
