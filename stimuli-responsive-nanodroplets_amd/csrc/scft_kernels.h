@@ -1535,6 +1535,122 @@ __global__ void __launch_bounds__((64 * FReg2<B, P>::W)) k_fftreg2(const LineOp 
     }
 }
 
+// ------------------------- one launch for the x lines of lengths 192, 256, 320
+//
+// A batch of uneven NX (the phase-diagram sweep: NX 192, 256 and 320) has
+// one periodic last-axis transform per length in each contour step.
+// k_fftmix runs them as one launch: workgroups [wg0, wg0 + nwg) of group k
+// run the persistent-wave loop of that length's register FFT (the bodies of
+// k_fftreg<16> and k_fftreg2<12,16> / <20,16>) over the group's ops, the
+// workgroups split between the groups in proportion to their work per
+// wave.  Each launch alone would take every CU and end in a partial round
+// of units; side by side on streams they overlapped only in part.  LDS is
+// the largest of the three layouts (the 256-point tiles), so two
+// workgroups share a CU as in each kernel alone; the 320-point group runs
+// three of the four waves, as k_fftreg2<20,16> does.
+
+struct MixGroup {
+    int op0;        // first op of the group, from the launch's first op
+    int nunits;     // wave units of the group
+    int upo;        // units per op
+    int wg0, nwg;   // the group's workgroups
+};
+
+struct MixArgs {
+    MixGroup g[3];  // lengths 256, 192, 320 (nwg 0: absent)
+};
+
+constexpr int kMixThreads = 64 * kFRWaves;
+
+template <int... N>
+constexpr int max_of() { int m = 0; ((m = N > m ? N : m), ...); return m; }
+
+constexpr int kMixLds = max_of<int((kFRWaves * FReg<16>::G * FReg<16>::RG + FReg<16>::L + FReg<16>::L / 16) *
+                                   sizeof(cpx<double>)),
+                               int((FReg2<12>::W * FReg2<12>::G * FReg2<12>::RG + FReg2<12>::L + FReg2<12>::L / 16) *
+                                   sizeof(cpx<double>)),
+                               int((FReg2<20>::W * FReg2<20>::G * FReg2<20>::RG + FReg2<20>::L + FReg2<20>::L / 16) *
+                                   sizeof(cpx<double>))>();
+static_assert(FReg2<12>::W <= kFRWaves && FReg2<20>::W <= kFRWaves, "mixed launch has four waves per workgroup");
+
+// k_fftreg<16>'s loop over the units of workgroup vb of nvb.
+template <typename Tin, typename Tout>
+__device__ __forceinline__ void mix_run256(const LineOp *__restrict__ ops, int nunits, int upo, int vb, int nvb,
+                                           cpx<double> *smem)
+{
+    using F = FReg<16>;
+    constexpr int G = F::G, RG = F::RG;
+    cpx<double> *T = smem, *TW = smem + kFRWaves * G * RG;
+    const int tid = threadIdx.x, lane = tid & 63;
+    freg_stage_twiddles<16>(ops[0], TW, tid, kMixThreads);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane % G, p = lane / G;
+    cpx<double> *Tg = T + (wave * G + g) * RG;
+    const int ustride = nvb * kFRWaves;
+    int u = vb * kFRWaves + wave;
+    if (u >= nunits) return;   // no workgroup barrier follows
+    FRegLoads<Tin> ld;
+    freg_fetch<16, Tin>(ops, upo, u, nunits, g, p, ld);
+    for (; u < nunits; u += ustride) {
+        const LineOp &op = ops[u / upo];
+        const int r0 = (u % upo) * F::UNIT + 2 * g;
+        cpx<double> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = cpx<double>{double(ld.x[m].x), double(ld.x[m].y)};
+        freg_fetch<16, Tin>(ops, upo, u + ustride, nunits, g, p, ld);
+        freg_apply<16, Tout>(op, r0, v, Tg, TW, p);
+    }
+}
+
+// k_fftreg2<B, 16>'s loop over the units of workgroup vb of nvb (waves
+// W and up idle after the twiddle barrier).
+template <int B, typename Tin, typename Tout>
+__device__ __forceinline__ void mix_run2(const LineOp *__restrict__ ops, int nunits, int upo, int vb, int nvb,
+                                         cpx<double> *smem)
+{
+    using F = FReg2<B, 16>;
+    constexpr int L = F::L, G = F::G, RG = F::RG, W = F::W;
+    cpx<double> *T = smem, *TW = smem + W * G * RG;
+    const int tid = threadIdx.x, lane = tid & 63;
+    stage_twiddles<L>(ops[0], TW, tid, kMixThreads);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (wave >= W) return;   // no workgroup barrier follows
+    const int g = lane % G, p = lane / G;
+    cpx<double> *Tg = T + (wave * G + g) * RG;
+    const int ustride = nvb * W;
+    int u = vb * W + wave;
+    if (u >= nunits) return;
+    FReg2Loads<B, Tin> ld;
+    freg2_fetch<B, 16, Tin>(ops, upo, u, nunits, g, p, ld);
+    for (; u < nunits; u += ustride) {
+        const LineOp &op = ops[u / upo];
+        const int r0 = (u % upo) * F::UNIT + 2 * g;
+        cpx<double> v[B];
+#pragma unroll
+        for (int m = 0; m < B; ++m) v[m] = cpx<double>{double(ld.x[m].x), double(ld.x[m].y)};
+        freg2_fetch<B, 16, Tin>(ops, upo, u + ustride, nunits, g, p, ld);
+        freg2_apply<B, 16, Tout>(op, r0, v, Tg, TW, p);
+    }
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kMixThreads) k_fftmix(const LineOp *__restrict__ ops, MixArgs a)
+{
+    __shared__ cpx<double> smem[kMixLds / sizeof(cpx<double>)];
+    const int b = blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const MixGroup &gr = a.g[k];
+        if (b < gr.wg0 || b >= gr.wg0 + gr.nwg) continue;
+        if (k == 0) mix_run256<Tin, Tout>(ops + gr.op0, gr.nunits, gr.upo, b - gr.wg0, gr.nwg, smem);
+        else if (k == 1) mix_run2<12, Tin, Tout>(ops + gr.op0, gr.nunits, gr.upo, b - gr.wg0, gr.nwg, smem);
+        else mix_run2<20, Tin, Tout>(ops + gr.op0, gr.nunits, gr.upo, b - gr.wg0, gr.nwg, smem);
+        return;
+    }
+}
+
 // ------------------------------------------- fused step on a periodic 64 x 64 plane
 //
 // One contour step of one chain on a periodic 2D plane (NY = 1) of 64 x 64

@@ -217,9 +217,9 @@ struct Slot {
 
 enum LaunchKind {
     LK_LINEOP = 0, LK_DENSE = 1, LK_FFTWAVE = 2, LK_DENSE_MFMA = 3, LK_DENSE_MFMA2 = 4, LK_FFTREG = 5, LK_FFTREG2 = 6,
-    LK_PLANE = 7, LK_FILM = 8, LK_FFTX = 9
+    LK_PLANE = 7, LK_FILM = 8, LK_FFTX = 9, LK_FFTMIX = 10
 };
-constexpr int kLaunchKinds = 10;
+constexpr int kLaunchKinds = 11;
 
 struct Launch {
     int kind;        // LaunchKind
@@ -231,6 +231,7 @@ struct Launch {
     FilmTab ftab{};  // tables of a fused film step (LK_FILM)
     int step = 0;    // contour step
     bool pf = false; // LK_DENSE_MFMA2: the kernel form with the next tile's slice rows in flight
+    MixArgs mix{};   // LK_FFTMIX: the length groups
 };
 
 bool fftwave_supported(int L)
@@ -255,6 +256,7 @@ struct scft_ctx {
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> evs;
     bool par_axes = true;            // SCFT_PARX=0: one after another
+    bool xmix = true;                // one launch for the x lengths 192/256/320 of a step (SCFT_XMIX=0: off)
     bool zpf = true;                 // z operator with the next tile's slice rows in flight (SCFT_ZPF=0: off)
     std::vector<Slot> slots;
     std::vector<void *> allocs;
@@ -742,6 +744,61 @@ bool film_fits(const scft_ctx *c)
     return c->NY == 1 && c->NZ == kFilmN && c->dense_mfma && !c->dense_mfma1 && !c->tmp_wide && c->zkind == Z_DENSE;
 }
 
+// The register-FFT launches of one step and axis for lengths 192, 256 and
+// 320 (from launches[lfirst] on, their ops consecutive) become one
+// LK_FFTMIX launch: the workgroups of two per CU are split between the
+// lengths in proportion to units x length / waves per workgroup, each group
+// at least one workgroup and at most one unit per wave.  FP32 slices and
+// scratch only: with FP64 lines the 320-point body needs more than 256
+// registers (one wave per SIMD), and the mixed kernel would hold the other
+// two lengths to that.
+int merge_mix(scft_ctx *c, size_t lfirst)
+{
+    std::vector<Launch> &Ls = c->launches;
+    const size_t n = Ls.size() - lfirst;
+    if (n < 2 || c->tsize != 4 || c->tmp_wide) return SCFT_OK;
+    auto group = [](const Launch &ln) {
+        if (ln.kind == LK_FFTREG && ln.L == 256) return 0;
+        if (ln.kind == LK_FFTREG2 && ln.L == 192) return 1;
+        if (ln.kind == LK_FFTREG2 && ln.L == 320) return 2;
+        return -1;
+    };
+    const int waves[3] = {kFRWaves, FReg2<12>::W, FReg2<20>::W};
+    double w[3] = {0.0, 0.0, 0.0}, tot = 0.0;
+    int units[3] = {0, 0, 0};
+    Launch m{};
+    m.kind = LK_FFTMIX;
+    m.pos = Ls[lfirst].pos;
+    m.first = Ls[lfirst].first;
+    m.count = 0;
+    for (size_t i = lfirst; i < Ls.size(); ++i) {
+        const int k = group(Ls[i]);
+        if (k < 0 || Ls[i].first != m.first + m.count) return SCFT_OK;   // not a mix of these lengths
+        MixGroup &gr = m.mix.g[k];
+        gr.op0 = Ls[i].first - m.first;
+        gr.upo = Ls[i].P;
+        gr.nunits = Ls[i].count * Ls[i].P;
+        units[k] = gr.nunits;
+        w[k] = double(gr.nunits) * Ls[i].L / waves[k];
+        tot += w[k];
+        m.count += Ls[i].count;
+        m.L = std::max(m.L, Ls[i].L);
+    }
+    int wg = 0;
+    for (int k = 0; k < 3; ++k) {
+        MixGroup &gr = m.mix.g[k];
+        gr.wg0 = wg;
+        if (units[k] == 0) continue;
+        const int most = (units[k] + waves[k] - 1) / waves[k];
+        gr.nwg = std::max(1, std::min(most, int(std::lround(2.0 * c->ncu * w[k] / tot))));
+        wg += gr.nwg;
+    }
+    m.gx = wg;
+    Ls.resize(lfirst);
+    Ls.push_back(m);
+    return SCFT_OK;
+}
+
 int build_schedule(scft_ctx *c)
 {
     drop_graph(c);
@@ -959,6 +1016,7 @@ int build_schedule(scft_ctx *c)
                 c->launches.push_back(ln);
             }
             // one launch per transform length (slots of a batch may differ in NX)
+            const size_t lfirst = c->launches.size();
             std::vector<int> Ls;
             for (auto &op : tl)
                 if (std::find(Ls.begin(), Ls.end(), op.L) == Ls.end()) Ls.push_back(op.L);
@@ -1023,6 +1081,7 @@ int build_schedule(scft_ctx *c)
                 }
                 c->launches.push_back(ln);
             }
+            if (c->xmix) RETCHK(merge_mix(c, lfirst));
         }
         if (t == 0) c->nstep0 = int(c->launches.size());
     }
@@ -1226,6 +1285,10 @@ void launch_one(scft_ctx *c, const Launch &ln, hipStream_t st)
         else hipLaunchKernelGGL((k_fftx<320, Tin>), g1, b, 0, st, ops, ln.P);
         break;
     }
+    case LK_FFTMIX:
+        hipLaunchKernelGGL((k_fftmix<Tin, Tout>), dim3(ln.gx), dim3(kMixThreads), 0, st, c->lineops + ln.first,
+                           ln.mix);
+        break;
     case LK_FFTREG:
         if (ln.L == 256)
             hipLaunchKernelGGL((k_fftreg<16, Tin, Tout>), dim3(ln.gx), dim3(64 * kFRWaves), 0, st,
@@ -1759,6 +1822,8 @@ int scft_batch_create(const scft_params *proto, int nslots, int max_nx, int prec
     }
     const char *px = std::getenv("SCFT_PARX");
     c->par_axes = !(px && px[0] == '0');
+    const char *xm = std::getenv("SCFT_XMIX");
+    c->xmix = !(xm && xm[0] == '0');
     // SCFT_ZPF=0: the MFMA z operator without the next tile's slice rows in
     // flight (k_zdense_mfma2 instead of k_zdense_mfma2p)
     const char *zp = std::getenv("SCFT_ZPF");
@@ -2263,7 +2328,7 @@ int scft_time_launches(scft_ctx *c, int kind, int reps, double *avg_us, double *
         const bool is_z = (ln.kind == LK_DENSE || ln.kind == LK_DENSE_MFMA || ln.kind == LK_DENSE_MFMA2) ||
                           ln.kind == LK_PLANE || ln.kind == LK_FILM ||
                           ((ln.kind == LK_LINEOP || ln.kind == LK_FFTWAVE || ln.kind == LK_FFTREG ||
-                            ln.kind == LK_FFTREG2 || ln.kind == LK_FFTX) && ln.pos == 0);
+                            ln.kind == LK_FFTREG2 || ln.kind == LK_FFTX || ln.kind == LK_FFTMIX) && ln.pos == 0);
         if ((kind == 0) != is_z) continue;
         if (kind == 1 && ln.pos != 2 && ln.pos != 3) continue;
         sel.push_back(ln);
@@ -2357,10 +2422,10 @@ int scft_get_schedule(const scft_ctx *c, char *buf, int len)
     }
     std::snprintf(buf, len,
                   "z=%s launches/iter: lineop=%d dense=%d fftwave=%d fftreg=%d dense_mfma=%d dense_mfma2=%d graph=%d "
-                  "fftreg2=%d plane=%d film=%d fftx=%d mfma2_pf=%d",
+                  "fftreg2=%d plane=%d film=%d fftx=%d mfma2_pf=%d fftmix=%d",
                   c->zkind == Z_DENSE ? "dense" : "fft", n[LK_LINEOP], n[LK_DENSE], n[LK_FFTWAVE], n[LK_FFTREG],
                   n[LK_DENSE_MFMA], n[LK_DENSE_MFMA2], c->use_graph ? 1 : 0, n[LK_FFTREG2], n[LK_PLANE],
-                  n[LK_FILM], n[LK_FFTX], npf);
+                  n[LK_FILM], n[LK_FFTX], npf, n[LK_FFTMIX]);
     return SCFT_OK;
 }
 

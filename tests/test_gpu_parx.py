@@ -38,7 +38,10 @@ class _env:
 
 
 def _run(abi, configs, lib, protos, precision, parx, film):
-    with _env(SCFT_PARX=parx, SCFT_ZKIND="dense", SCFT_FILM="1" if film else "0", SCFT_XFFT="reg"):
+    # SCFT_XMIX=0: one x launch per length (FP32-split otherwise runs the
+    # three lengths as one launch, tests/test_gpu_xmix.py)
+    with _env(SCFT_PARX=parx, SCFT_ZKIND="dense", SCFT_FILM="1" if film else "0", SCFT_XFFT="reg",
+              SCFT_XMIX="0"):
         b = abi.Batch(lib, protos[0], len(protos), max_nx=320, precision=precision)
     for s, p in enumerate(protos):
         b.init_slot(s, p)
