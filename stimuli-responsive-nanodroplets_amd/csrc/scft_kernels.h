@@ -15,7 +15,8 @@
 //     ~6x that.  The autotuner picks per size.
 //   - k_fftreg<P>: periodic lines of length 16 P (128, 256) as a four-step
 //     FFT in registers; k_fftreg2<B>: the same for 16 B (192, 320), 16
-//     lanes of B points; k_fftwave: other compile-time lengths, one
+//     lanes of B points; k_fftmix: the lengths 192, 256 and 320 of a batch
+//     of uneven NX in one launch; k_fftwave: other compile-time lengths, one
 //     wavefront per line; k_lineop: any length, LDS Stockham passes, and the
 //     reference's Dirichlet transform (odd extension to N = 2(NZ+1),
 //     SCFT.c:2551-2592) when the dense z operator is not used.
