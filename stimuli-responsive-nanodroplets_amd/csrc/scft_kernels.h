@@ -794,8 +794,39 @@ __device__ __forceinline__ void mfma2_tile(const DenseOp &op, int rbase, Tout *d
     mfma2_mma_store<Tout, HSP>(op, rbase, dst, Ms, Ma, Xw, lane, sf, af);
 }
 
-template <typename Tin, typename Tout, int HSP>
-__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp *__restrict__ ops, int ntiles, int tpo)
+// The 16-row tiles of a z launch.  A batch of uneven NX has ops of
+// different row counts: the ops come grouped by row count, group g's from
+// op op0[g] on with tpo[g] tiles each and its tiles numbered from t0[g], so
+// that no tile lies past its op's last row (with one tile count for every
+// op, the shorter ops' spare tiles ran the whole tile for nothing).
+constexpr int kZGroups = 4;
+
+struct ZTiles {
+    int ng;
+    int op0[kZGroups], tpo[kZGroups], t0[kZGroups];
+};
+
+// Tile t: op index and first row (wave-uniform).  GR false: one group (every
+// op with the same row count), the plain division.
+template <bool GR>
+__device__ __forceinline__ void ztile(const ZTiles &z, int t, int &opi, int &rbase)
+{
+    if constexpr (!GR) {
+        opi = t / z.tpo[0];
+        rbase = (t % z.tpo[0]) * 16;
+        return;
+    }
+    int g = 0;
+#pragma unroll
+    for (int k = 1; k < kZGroups; ++k) g = (k < z.ng && t >= z.t0[k]) ? k : g;
+    const int rel = t - z.t0[g];
+    opi = z.op0[g] + rel / z.tpo[g];
+    rbase = (rel % z.tpo[g]) * 16;
+}
+
+template <typename Tin, typename Tout, int HSP, bool GR>
+__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp *__restrict__ ops, int ntiles,
+                                                                   ZTiles zt)
 {
     constexpr int LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
     __shared__ double Ms[HSP * LDM];
@@ -807,8 +838,10 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: tile indices stay scalar
     double *Xw = X + wave * 8 * XS;
     for (int t = blockIdx.x * kMfma2Waves + wave; t < ntiles; t += gridDim.x * kMfma2Waves) {
-        const DenseOp &op = ops[t / tpo];
-        mfma2_tile<Tin, Tout, HSP>(op, (t % tpo) * 16, static_cast<Tout *>(op.dst), Ms, Ma, Xw, lane);
+        int opi, rbase;
+        ztile<GR>(zt, t, opi, rbase);
+        const DenseOp &op = ops[opi];
+        mfma2_tile<Tin, Tout, HSP>(op, rbase, static_cast<Tout *>(op.dst), Ms, Ma, Xw, lane);
     }
 }
 
@@ -818,8 +851,9 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2(const DenseOp
 // makes the compiler wait for the loads early).  FP32 rows of n = N points
 // (N a multiple of 16, at most 2 HSP) in whole 16-row tiles and the FP64
 // factor only (the host checks).
-template <typename Tin, int HSP, int N>
-__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2p(const DenseOp *__restrict__ ops, int ntiles, int tpo)
+template <typename Tin, int HSP, int N, bool GR>
+__global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2p(const DenseOp *__restrict__ ops, int ntiles,
+                                                                    ZTiles zt)
 {
     constexpr int LDM = Mfma2<HSP>::LDM, XS = Mfma2<HSP>::XS;
     __shared__ double Ms[HSP * LDM];
@@ -836,8 +870,10 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2p(const DenseO
     // by every step of a block, is loaded at staging
     Pair<Tin> xn[16];
     auto load_x = [&](int u) {
-        const DenseOp &o = ops[u / tpo];
-        const int rb = (u % tpo) * 16, n = o.n, nr = min(16, o.nrows - rb);
+        int oi, rb;
+        ztile<GR>(zt, u, oi, rb);
+        const DenseOp &o = ops[oi];
+        const int n = o.n, nr = min(16, o.nrows - rb);
         const unsigned o0 = unsigned(rb) * unsigned(n) + unsigned(2 * lane < n ? 2 * lane : 0);
         const Tin *src = static_cast<const Tin *>(o.src);
 #pragma unroll
@@ -845,8 +881,9 @@ __global__ void __launch_bounds__(64 * kMfma2Waves) k_zdense_mfma2p(const DenseO
     };
     load_x(t);
     for (; t < ntiles; t += gridDim.x * kMfma2Waves) {
-        const DenseOp &op = ops[t / tpo];
-        const int rbase = (t % tpo) * 16;
+        int opi, rbase;
+        ztile<GR>(zt, t, opi, rbase);
+        const DenseOp &op = ops[opi];
         Z2Regs<Tin> R;
         {
             const int n = op.n, nr = min(16, op.nrows - rbase);

@@ -232,6 +232,8 @@ struct Launch {
     int step = 0;    // contour step
     bool pf = false; // LK_DENSE_MFMA2: the kernel form with the next tile's slice rows in flight
     MixArgs mix{};   // LK_FFTMIX: the length groups
+    ZTiles zt{};     // LK_DENSE_MFMA2: tiles of the ops grouped by row count
+    int ntiles = 0;  // LK_DENSE_MFMA2: tiles of the launch
 };
 
 bool fftwave_supported(int L)
@@ -1006,7 +1008,34 @@ int build_schedule(scft_ctx *c)
                     ln.kind = even ? LK_DENSE_MFMA2 : LK_DENSE_MFMA;
                     ln.hsp = hsp;
                     ln.P = (maxl + 15) / 16;   // tiles per op
-                    const int waves = ln.count * ln.P;
+                    int waves = ln.count * ln.P;
+                    if (even) {
+                        // ops grouped by row count (uneven NX), each with its own tiles
+                        std::stable_sort(td.begin(), td.end(),
+                                         [](const DenseOp &a, const DenseOp &b) { return a.nrows > b.nrows; });
+                        ZTiles zt{};
+                        int nt = 0;
+                        for (size_t i = 0; i < td.size();) {
+                            size_t j = i;
+                            while (j < td.size() && td[j].nrows == td[i].nrows) ++j;
+                            if (zt.ng == kZGroups) {   // more row counts than groups: one tile count for all
+                                zt = ZTiles{};
+                                zt.ng = 1;
+                                zt.tpo[0] = ln.P;
+                                nt = ln.count * ln.P;
+                                break;
+                            }
+                            zt.op0[zt.ng] = int(i);
+                            zt.tpo[zt.ng] = (td[i].nrows + 15) / 16;
+                            zt.t0[zt.ng] = nt;
+                            nt += int(j - i) * zt.tpo[zt.ng];
+                            ++zt.ng;
+                            i = j;
+                        }
+                        ln.zt = zt;
+                        ln.ntiles = nt;
+                        waves = nt;
+                    }
                     ln.gx = std::max(1, std::min((waves + wpg - 1) / wpg, c->ncu));
                 } else {
                     ln.kind = LK_DENSE;
@@ -1241,19 +1270,30 @@ void launch_one(scft_ctx *c, const Launch &ln, hipStream_t st)
     case LK_DENSE_MFMA2: {
         const DenseOp *ops = c->denseops + ln.first;
         const dim3 g1(ln.gx), b(64 * kMfma2Waves);
-        const int nt = ln.count * ln.P;
-        if constexpr (sizeof(Tout) == 4) {
-            if (ln.pf) {
-                if (ln.hsp == 64) hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 64, 128>), g1, b, 0, st, ops, nt, ln.P);
-                else if (ln.L == 64) hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32, 64>), g1, b, 0, st, ops, nt, ln.P);
-                else hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32, 48>), g1, b, 0, st, ops, nt, ln.P);
-                break;
+        const int nt = ln.ntiles;
+        const ZTiles &zt = ln.zt;
+        // the grouped tile map only where row counts differ (it costs a
+        // little on a uniform batch)
+        auto go = [&](auto gr) {
+            constexpr bool GR = decltype(gr)::value;
+            if constexpr (sizeof(Tout) == 4) {
+                if (ln.pf) {
+                    if (ln.hsp == 64)
+                        hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 64, 128, GR>), g1, b, 0, st, ops, nt, zt);
+                    else if (ln.L == 64)
+                        hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32, 64, GR>), g1, b, 0, st, ops, nt, zt);
+                    else
+                        hipLaunchKernelGGL((k_zdense_mfma2p<Tin, 32, 48, GR>), g1, b, 0, st, ops, nt, zt);
+                    return;
+                }
             }
-        }
-        if (ln.hsp == 16) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 16>), g1, b, 0, st, ops, nt, ln.P);
-        else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 32>), g1, b, 0, st, ops, nt, ln.P);
-        else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 48>), g1, b, 0, st, ops, nt, ln.P);
-        else hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 64>), g1, b, 0, st, ops, nt, ln.P);
+            if (ln.hsp == 16) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 16, GR>), g1, b, 0, st, ops, nt, zt);
+            else if (ln.hsp == 32) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 32, GR>), g1, b, 0, st, ops, nt, zt);
+            else if (ln.hsp == 48) hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 48, GR>), g1, b, 0, st, ops, nt, zt);
+            else hipLaunchKernelGGL((k_zdense_mfma2<Tin, Tout, 64, GR>), g1, b, 0, st, ops, nt, zt);
+        };
+        if (zt.ng > 1) go(std::true_type{});
+        else go(std::false_type{});
         break;
     }
     case LK_DENSE:
