@@ -1,10 +1,34 @@
-"""Shared inputs of the writer-format checks (tests/golden/writers_expected.json)."""
+"""Shared helpers of the tests: the library's environment switches set for
+a block, and the inputs of the writer-format checks
+(tests/golden/writers_expected.json)."""
+import contextlib
 import json
 import os
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@contextlib.contextmanager
+def env(**kw):
+    """Environment variables set for the block (None: unset), as the library
+    reads them when a context or batch is created; the previous values are
+    restored afterwards, also when the block raises."""
+    old = {k: os.environ.get(k) for k in kw}
+    try:
+        for k, v in kw.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+        yield
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 def writers_fixture():

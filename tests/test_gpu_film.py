@@ -14,33 +14,16 @@ chain's plane, the scratch between the axes in LDS) against the split step
 - against the oracle's FP64 fixtures: C2 in test_gpu_parity.py and the C4
   sweep batch in test_gpu_configs.py run with the film step forced too.
 """
-import os
-
 import numpy as np
 import pytest
+
+from _cases import env as _env
 
 pytestmark = pytest.mark.gpu
 
 
 def _counts(sched):
     return dict((k, int(v)) for k, v in (kv.split("=") for kv in sched.split() if "=" in kv and kv[0] != "z"))
-
-
-class _env:
-    def __init__(self, **kw):
-        self.kw = kw
-
-    def __enter__(self):
-        self.old = {k: os.environ.get(k) for k in self.kw}
-        for k, v in self.kw.items():
-            os.environ[k] = v
-
-    def __exit__(self, *a):
-        for k, v in self.old.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
 
 
 def _run(abi, configs, lib, protos, precision, film, iters=(2, 3, 4), e2f="0"):

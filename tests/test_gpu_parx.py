@@ -9,32 +9,12 @@ and joined back into the batch's stream (SCFT_PARX=0: one after another).
 - the FP64 results also match the oracle (the C4 sweep fixtures run in
   test_gpu_configs.py with the default, side by side).
 """
-import os
-
 import numpy as np
 import pytest
 
+from _cases import env as _env
+
 pytestmark = pytest.mark.gpu
-
-
-class _env:
-    def __init__(self, **kw):
-        self.kw = kw
-
-    def __enter__(self):
-        self.old = {k: os.environ.get(k) for k in self.kw}
-        for k, v in self.kw.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
-
-    def __exit__(self, *a):
-        for k, v in self.old.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
 
 
 def _run(abi, configs, lib, protos, precision, parx, film):

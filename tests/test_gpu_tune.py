@@ -10,10 +10,10 @@ the register or wave-per-line FFT, per size class of the active set).
 Each case uses its own contour length, so that no earlier context's
 tuning result (cached per shape for the process) is reused.
 """
-import os
-
 import numpy as np
 import pytest
+
+from _cases import env as _env
 
 pytestmark = pytest.mark.gpu
 
@@ -22,26 +22,6 @@ TOL = 1e-10
 
 def _srel(a, b):
     return abs(a - b) / max(abs(b), 1.0)
-
-
-class _env:
-    def __init__(self, **kw):
-        self.kw = kw
-
-    def __enter__(self):
-        self.old = {k: os.environ.get(k) for k in self.kw}
-        for k, v in self.kw.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
-
-    def __exit__(self, *a):
-        for k, v in self.old.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
 
 
 def _tune_lines(err):

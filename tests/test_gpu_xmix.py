@@ -14,36 +14,16 @@ length).
 The full 192-slot C4 batch runs the mixed launch in
 test_gpu_configs.py::test_c4_sweep_batch_matches_golden (fp32, split).
 """
-import os
-
 import numpy as np
 import pytest
+
+from _cases import env as _env
 
 pytestmark = pytest.mark.gpu
 
 
 def _counts(sched):
     return {k: int(v) for k, v in (kv.split("=") for kv in sched.split() if "=" in kv and kv[0] != "z")}
-
-
-class _env:
-    def __init__(self, **kv):
-        self.kv = kv
-
-    def __enter__(self):
-        self.old = {k: os.environ.get(k) for k in self.kv}
-        for k, v in self.kv.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
-
-    def __exit__(self, *exc):
-        for k, v in self.old.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
 
 
 def _rel(a, b):
